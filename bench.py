@@ -1,0 +1,198 @@
+#!/usr/bin/env python3
+"""Photon-differential splatting benchmark: photons/s of one forward + backward
+pass on one MI355X, with the dominant kernel's HBM roofline fraction and a
+multithreaded host-CPU baseline of the same formulae.
+
+Default workload (the caustic-like large splat): 2^22 photons into a
+3 x 1024 x 1024 fp32 image, max_pixel_radius 12, caustic-line positions,
+anisotropic differentials, N(0, 1) upstream gradient; inputs resident in HBM
+before the timed region.  One "step" = pds_forward + pds_backward of the whole
+batch.  Prints ONE JSON line on stdout; progress goes to stderr.
+
+    python bench.py [--steps K] [--warmup W] [--mode auto|plain|binned|deterministic]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+
+# Algorithmic bytes (reference tensor layout, fp32 + int64 channel):
+#   forward : reads E 4 + xp 8 + M 24 + cp 8 + radius 4 = 48 B per photon, writes 4 B per pixel
+#   backward: reads the same 48 B, writes gE 4 + gx 8 + gM 24 = 36 B per photon, reads 4 B per pixel
+FWD_B_PER_PHOTON, BWD_B_PER_PHOTON, B_PER_PIXEL_PASS = 48, 84, 4
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_workload(name):
+    from shapefromcaustics_amd import workloads as WL
+    if name == "caustic":
+        p = WL.caustic()
+        desc = "caustic: 2^22 photons -> 3x1024x1024 fp32, cap 12, 64 caustic lines + 20% uniform"
+    elif name == "plain":
+        p = WL.plain()
+        desc = "plain: 2^18 photons -> 1x256x256 fp32, cap 8, uniform"
+    elif name == "batched":
+        p = WL.batched()
+        desc = "batched: 16 ragged splats (logU(2^18,2^22) photons) -> 16x512x512 fp32, cap 8"
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    return p, desc
+
+
+def cpu_baseline(p, grad, budget_s=12.0):
+    """OpenMP FP32 host implementation of the same formulae on the same inputs."""
+    from oracle import pds_oracle as O
+    threads = O.cpu_threads()
+    times = []
+    t_start = time.perf_counter()
+    while len(times) < 30 and (time.perf_counter() - t_start) < budget_s:
+        t0 = time.perf_counter()
+        O.cpu_forward_f32(*p.arrays(), p.size, p.cap)
+        O.cpu_backward_f32(grad, *p.arrays(), p.cap)
+        times.append(time.perf_counter() - t0)
+    times.sort()
+    t = times[len(times) // 2]
+    return {"value": p.n / t, "unit": "photons/s", "cores": threads, "kind": "port",
+            "sample": f"the full benchmark batch ({p.n} photons, fwd+bwd), median of {len(times)} runs "
+                      f"on {threads} OpenMP threads of the host"}
+
+
+def load_traffic(kernel, workload):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        e = d.get(workload, {}).get(kernel)
+        return None if e is None else float(e["bytes_per_launch"])
+    except (OSError, ValueError, KeyError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="caustic")
+    ap.add_argument("--mode", default="auto")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the host-CPU baseline")
+    ap.add_argument("--no-plain", action="store_true", help="skip timing the plain-atomic comparator")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (photon sharding plus one image "
+              "all-reduce is the design, not built); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from shapefromcaustics_amd import workloads as WL
+    from shapefromcaustics_amd.PhotonDifferentialSplatting import pds_backward, pds_forward
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    log(f"[bench] generating workload {args.workload}")
+    p, desc = make_workload(args.workload)
+    grad_np = WL.upstream_grad(p.size, seed=2)
+    E, xp, M, cp, radius = p.to_torch(dev)
+    grad = torch.from_numpy(grad_np).to(dev)
+    C, H, W = p.size
+    N = p.n
+    torch.cuda.synchronize()
+
+    def step(mode, ev=None):
+        if ev is not None:
+            ev[0].record()
+        img = pds_forward(E, xp, M, cp, radius, p.size, p.cap, mode=mode)[0]
+        if ev is not None:
+            ev[1].record()
+        g = pds_backward(grad, E, xp, M, cp, radius, p.cap, mode=mode)
+        if ev is not None:
+            ev[2].record()
+        return img, g
+
+    def timed(mode, steps, warmup):
+        for _ in range(warmup):
+            step(mode)
+        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(steps):
+            step(mode, evs[k])
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / steps
+        fwd = sorted(e[0].elapsed_time(e[1]) for e in evs)
+        bwd = sorted(e[1].elapsed_time(e[2]) for e in evs)
+        return wall, fwd[len(fwd) // 2] * 1e-3, bwd[len(bwd) // 2] * 1e-3
+
+    log(f"[bench] {desc}; mode={args.mode}; warmup {args.warmup}, steps {args.steps}")
+    wall, t_fwd, t_bwd = timed(args.mode, args.steps, args.warmup)
+    log(f"[bench] {args.mode}: {wall * 1e3:.3f} ms/step (fwd {t_fwd * 1e6:.1f} us, bwd {t_bwd * 1e6:.1f} us)")
+
+    plain = None
+    if not args.no_plain and args.mode != "plain":
+        pw, pf, pb = timed("plain", max(3, args.steps // 4), 2)
+        plain = {"value": N / pw, "ms_per_step": pw * 1e3, "fwd_us": pf * 1e6, "bwd_us": pb * 1e6}
+        log(f"[bench] plain-atomic comparator: {pw * 1e3:.3f} ms/step")
+
+    # dominant pass: the backward is one kernel launch; the forward's kernels
+    # are bracketed together (zero fill + splat, or binning + tile passes)
+    pix = C * H * W
+    fwd_bytes = FWD_B_PER_PHOTON * N + B_PER_PIXEL_PASS * pix
+    bwd_bytes = BWD_B_PER_PHOTON * N + B_PER_PIXEL_PASS * pix
+    if t_bwd >= t_fwd:
+        kname, kbytes, kt = "pds_backward", bwd_bytes, t_bwd
+    else:
+        kname, kbytes, kt = "pds_forward", fwd_bytes, t_fwd
+    achieved = kbytes / kt / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(kname, args.workload),
+                "kernel": kname, "algorithmic_bytes": kbytes, "avg_duration_us": round(kt * 1e6, 2),
+                "step_achieved_GBps": round((fwd_bytes + bwd_bytes) / wall / 1e9, 1)}
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline (OpenMP FP32, same inputs)")
+        cpu = cpu_baseline(p, grad_np)
+        log(f"[bench] CPU: {cpu['value']:.3e} photons/s on {cpu['cores']} threads")
+
+    line = {
+        "metric": "photons/sec fwd+bwd @2^22 photons->1024^2 (% HBM roofline), batched vs plain",
+        "value": N / wall,
+        "unit": "photons/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded numpy PCG64 photons and N(0,1) upstream gradient)",
+        "config": {"workload": desc, "photons": N, "image": [C, H, W], "max_pixel_radius": p.cap,
+                   "mode": args.mode, "parallelism": "single GPU"},
+        "fwd_us": round(t_fwd * 1e6, 2),
+        "bwd_us": round(t_bwd * 1e6, 2),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "plain_atomic": plain,
+        "speedup_vs_plain": None if plain is None else round(plain["ms_per_step"] / (wall * 1e3), 3),
+        "speedup_vs_cpu": None if cpu is None else round((N / wall) / cpu["value"], 1),
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

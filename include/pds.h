@@ -1,0 +1,98 @@
+/*
+ * pds.h -- C ABI of the MI355X (gfx950) photon-differential splatting library
+ * (libpds_hip.so).
+ *
+ * The library replaces the reference's CUDA extension ``PhotonDifferentialSplatting``:
+ *   PyOptix/PhotonDifferentialSplattig.cpp:12-25   pds_forward  (checks + launch)
+ *   PyOptix/PhotonDifferentialSplattig.cpp:27-32   pds_backward (checks + launch)
+ *   PyOptix/kernel/photon_differentials.cu:115-134 pds_cuda_forward  (alloc + kernel)
+ *   PyOptix/kernel/photon_differentials.cu:223-247 pds_cuda_backward (alloc + kernel)
+ * Only plain pointers, sizes and an opaque stream cross this boundary; the
+ * Python binding (shapefromcaustics_amd/PhotonDifferentialSplatting.py) keeps
+ * the reference's tensor-level signatures on top of it.
+ *
+ * Conventions
+ *  - All arrays are device pointers in the reference's layout, contiguous:
+ *      Ep [N] f32, xp [2,N] f32, Mp [2,3,N] f32, cp [1,N] i64, radius [N] f32,
+ *      image / grad [C,H,W] f32, grad_Ep [N], grad_xp [2,N], grad_Mp [2,3,N].
+ *  - Buffers are caller-owned.  The forward zero-fills ``out`` itself.
+ *  - ``stream`` is a hipStream_t (NULL = the null stream); every launch is
+ *    stream-ordered on it and no call synchronises the host.
+ *  - ``workspace`` is caller-owned device scratch of at least
+ *    pds_workspace_bytes(...) bytes (it may be NULL when that size is 0).
+ *  - Return value: PDS_OK (0) or a negative pds_status; no exception crosses
+ *    the ABI.  pds_last_error() gives the message of the calling thread's
+ *    last failure.
+ *  - Deliberate supersets of the reference, all documented: negative channels
+ *    are skipped (the reference indexes out of bounds), skipped photons get
+ *    zero gradients (the reference leaves them uninitialised), the r = 0
+ *    gradient is its finite limit (the reference divides 0/0), photon counts
+ *    are 64-bit (the reference uses 32-bit accessors).
+ */
+#ifndef PDS_H
+#define PDS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum pds_status {
+    PDS_OK = 0,
+    PDS_ERR_ARGUMENT = -1,    /* invalid sizes, null pointers, bad mode */
+    PDS_ERR_WORKSPACE = -2,   /* workspace smaller than pds_workspace_bytes() */
+    PDS_ERR_HIP = -3,         /* a HIP runtime call or launch failed */
+    PDS_ERR_CAPACITY = -4     /* a size beyond a documented maximum */
+} pds_status;
+
+typedef enum pds_mode {
+    PDS_MODE_AUTO = 0,          /* fastest exact mode (binned forward, cooperative adjoint) */
+    PDS_MODE_PLAIN = 1,         /* one thread per photon, global atomics: the reference's schedule */
+    PDS_MODE_BINNED = 2,        /* tile-binned LDS forward, atomic flush (run-to-run order may vary) */
+    PDS_MODE_DETERMINISTIC = 3  /* tile-ordered forward, bitwise reproducible run to run */
+} pds_mode;
+
+/* Tuning knobs of the binned path; zero fields select the built-in default.
+ * None of them changes results beyond FP32 summation order: the semantic
+ * window (max_pixel_radius) is never altered. */
+typedef struct pds_tuning {
+    int32_t tile;          /* square screen tile edge in pixels: 8, 16, 32 or 64 */
+    int32_t seg_photons;   /* photon records per tile segment (one workgroup each) */
+    int32_t block_photons; /* photons per binning workgroup */
+    int32_t reserved[5];
+} pds_tuning;
+
+/* Bytes of device workspace the forward or backward of this problem needs
+ * in ``mode`` with tuning ``t`` (NULL = defaults). */
+size_t pds_workspace_bytes(int64_t N, int32_t C, int32_t H, int32_t W, int32_t max_pixel_radius,
+                           int32_t mode, const pds_tuning* t);
+
+/* Forward splat (reference pds_forward, PhotonDifferentialSplattig.cpp:12-25;
+ * kernel photon_differentials.cu:60-113).  out [C,H,W] is overwritten. */
+int pds_forward_f32(const float* Ep, const float* xp, const float* Mp, const int64_t* cp,
+                    const float* radius, int64_t N, int32_t C, int32_t H, int32_t W,
+                    int32_t max_pixel_radius, int32_t mode, const pds_tuning* t, float* out,
+                    void* workspace, size_t workspace_bytes, void* stream);
+
+/* Adjoint gather (reference pds_backward, PhotonDifferentialSplattig.cpp:27-32;
+ * kernel photon_differentials.cu:136-221).  Writes all of grad_Ep [N],
+ * grad_xp [2,N] and grad_Mp [2,3,N] (grad_Mp[:,2] = 0). */
+int pds_backward_f32(const float* grad, const float* Ep, const float* xp, const float* Mp,
+                     const int64_t* cp, const float* radius, int64_t N, int32_t C, int32_t H,
+                     int32_t W, int32_t max_pixel_radius, int32_t mode, const pds_tuning* t,
+                     float* grad_Ep, float* grad_xp, float* grad_Mp, void* workspace,
+                     size_t workspace_bytes, void* stream);
+
+/* Message of the calling thread's last failed call ("" if none). */
+const char* pds_last_error(void);
+
+/* Library version string. */
+const char* pds_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PDS_H */

@@ -1,0 +1,166 @@
+// pds_api.hip -- the C ABI of libpds_hip.so (declared in include/pds.h).
+//
+// Replaces the reference's pybind entry points pds_forward / pds_backward
+// (PyOptix/PhotonDifferentialSplattig.cpp:12-32) and their CUDA launchers
+// (PyOptix/kernel/photon_differentials.cu:115-134, :223-247) with plain
+// pointers, caller-owned buffers and an explicit stream.  Validation of
+// tensor-level properties (device, contiguity, the reference's shape
+// messages) happens in the Python binding; this layer validates sizes,
+// pointers, the mode and the workspace, and reports every failure through a
+// status code plus pds_last_error().
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+#include "pds.h"
+#include "pds_binned.hpp"
+#include "pds_internal.hpp"
+
+namespace {
+
+thread_local char g_last_error[512] = "";
+
+int fail(int code, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_last_error, sizeof(g_last_error), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* where) {
+    return fail(PDS_ERR_HIP, "%s: %s (%d)", where, hipGetErrorString(e), (int)e);
+}
+
+// Host twin of pds::window_ellipse_ok: the reference's window ellipse test
+// (photon_differentials.cu:93) is monotone in |x_off| and |y_off|, so if it
+// holds at (cap, cap) it holds for every offset of every window.
+bool ellipse_trivial(int32_t cap, int32_t W, int32_t H) {
+    if (cap < 0) return true;
+    const double w = (double)(float)W, h = (double)(float)H;
+    const double a = (double)(float)((int64_t)cap * cap) / (0.25 * w * w);
+    const double b = (double)(float)((int64_t)cap * cap) / (0.25 * h * h);
+    return a + b <= 1.0;
+}
+
+int check_common(int64_t N, int32_t C, int32_t H, int32_t W, int32_t mode, const void* E, const void* xp,
+                 const void* M, const void* cp, const void* radius) {
+    if (N < 0) return fail(PDS_ERR_ARGUMENT, "photon count N = %lld is negative", (long long)N);
+    if (C < 0 || H < 0 || W < 0)
+        return fail(PDS_ERR_ARGUMENT, "output size (%d, %d, %d) has a negative extent", C, H, W);
+    if ((int64_t)C * H * W >= ((int64_t)1 << 31))
+        return fail(PDS_ERR_CAPACITY, "output of %lld elements exceeds the 2^31 - 1 element limit",
+                    (long long)C * H * W);
+    if (mode < PDS_MODE_AUTO || mode > PDS_MODE_DETERMINISTIC) return fail(PDS_ERR_ARGUMENT, "unknown mode %d", mode);
+    if (N > 0 && (!E || !xp || !M || !cp || !radius))
+        return fail(PDS_ERR_ARGUMENT, "null photon input pointer with N = %lld", (long long)N);
+    return PDS_OK;
+}
+
+pds::PhotonArgs photon_args(const float* E, const float* xp, const float* M, const int64_t* cp, const float* radius,
+                            int64_t N) {
+    pds::PhotonArgs P;
+    P.E = E;
+    P.xp = xp;
+    P.M = M;
+    P.cp = cp;
+    P.radius = radius;
+    P.N = N;
+    return P;
+}
+
+pds::ImageArgs image_args(int32_t C, int32_t H, int32_t W, int32_t cap) {
+    pds::ImageArgs I;
+    I.C = C;
+    I.H = H;
+    I.W = W;
+    I.cap = cap;
+    I.fw = (float)W;
+    I.fh = (float)H;
+    I.ell_trivial = ellipse_trivial(cap, W, H) ? 1 : 0;
+    return I;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pds_last_error(void) { return g_last_error; }
+
+const char* pds_version(void) { return "pds_hip 0.1 gfx950"; }
+
+size_t pds_workspace_bytes(int64_t N, int32_t C, int32_t H, int32_t W, int32_t max_pixel_radius, int32_t mode,
+                           const pds_tuning* t) {
+    if (N < 0 || C <= 0 || H <= 0 || W <= 0) return 0;
+    if (mode == PDS_MODE_BINNED || mode == PDS_MODE_DETERMINISTIC || mode == PDS_MODE_AUTO) {
+        if (!ellipse_trivial(max_pixel_radius, W, H)) return 0;  // plain schedule, no workspace
+        return pds::binned_workspace_bytes(N, C, H, W, max_pixel_radius, mode, t);
+    }
+    return 0;
+}
+
+int pds_forward_f32(const float* Ep, const float* xp, const float* Mp, const int64_t* cp, const float* radius,
+                    int64_t N, int32_t C, int32_t H, int32_t W, int32_t max_pixel_radius, int32_t mode,
+                    const pds_tuning* t, float* out, void* workspace, size_t workspace_bytes, void* stream) {
+    g_last_error[0] = '\0';
+    int rc = check_common(N, C, H, W, mode, Ep, xp, Mp, cp, radius);
+    if (rc) return rc;
+    const int64_t numel = (int64_t)C * H * W;
+    if (numel == 0) return PDS_OK;
+    if (!out) return fail(PDS_ERR_ARGUMENT, "null output pointer");
+    const size_t need = pds_workspace_bytes(N, C, H, W, max_pixel_radius, mode, t);
+    if (need > 0 && (workspace == nullptr || workspace_bytes < need))
+        return fail(PDS_ERR_WORKSPACE, "workspace of %zu bytes is smaller than the %zu bytes required",
+                    workspace ? workspace_bytes : (size_t)0, need);
+    hipStream_t s = (hipStream_t)stream;
+    const pds::PhotonArgs P = photon_args(Ep, xp, Mp, cp, radius, N);
+    const pds::ImageArgs I = image_args(C, H, W, max_pixel_radius);
+    hipError_t e;
+    if (mode == PDS_MODE_PLAIN || !I.ell_trivial) {
+        if ((e = pds::launch_fill_zero(out, numel, s)) != hipSuccess) return hip_fail(e, "zero fill");
+        if ((e = pds::launch_forward_plain(P, I, out, s)) != hipSuccess) return hip_fail(e, "plain forward");
+        return PDS_OK;
+    }
+    rc = pds::binned_forward(P, I, mode, t, out, workspace, workspace_bytes, s);
+    if (rc == PDS_ERR_HIP) return hip_fail(hipGetLastError(), "binned forward");
+    if (rc) return fail(rc, "binned forward: %s", pds::binned_last_error());
+    return PDS_OK;
+}
+
+int pds_backward_f32(const float* grad, const float* Ep, const float* xp, const float* Mp, const int64_t* cp,
+                     const float* radius, int64_t N, int32_t C, int32_t H, int32_t W, int32_t max_pixel_radius,
+                     int32_t mode, const pds_tuning* t, float* grad_Ep, float* grad_xp, float* grad_Mp,
+                     void* workspace, size_t workspace_bytes, void* stream) {
+    g_last_error[0] = '\0';
+    (void)t;
+    (void)workspace;
+    (void)workspace_bytes;
+    int rc = check_common(N, C, H, W, mode, Ep, xp, Mp, cp, radius);
+    if (rc) return rc;
+    if (N == 0) return PDS_OK;
+    if (!grad_Ep || !grad_xp || !grad_Mp) return fail(PDS_ERR_ARGUMENT, "null gradient output pointer");
+    const int64_t numel = (int64_t)C * H * W;
+    if (numel > 0 && !grad) return fail(PDS_ERR_ARGUMENT, "null upstream gradient pointer");
+    hipStream_t s = (hipStream_t)stream;
+    const pds::PhotonArgs P = photon_args(Ep, xp, Mp, cp, radius, N);
+    const pds::ImageArgs I = image_args(C, H, W, max_pixel_radius);
+    const pds::GradArgs G = {grad_Ep, grad_xp, grad_Mp};
+    hipError_t e;
+    if (numel == 0) {
+        // every photon is skipped: all gradients are zero
+        if ((e = pds::launch_fill_zero(grad_Ep, N, s)) != hipSuccess) return hip_fail(e, "zero fill");
+        if ((e = pds::launch_fill_zero(grad_xp, 2 * N, s)) != hipSuccess) return hip_fail(e, "zero fill");
+        if ((e = pds::launch_fill_zero(grad_Mp, 6 * N, s)) != hipSuccess) return hip_fail(e, "zero fill");
+        return PDS_OK;
+    }
+    if (mode == PDS_MODE_PLAIN || !I.ell_trivial) {
+        if ((e = pds::launch_backward_plain(grad, P, I, G, s)) != hipSuccess) return hip_fail(e, "plain backward");
+        return PDS_OK;
+    }
+    if ((e = pds::launch_backward_flat(grad, P, I, G, s)) != hipSuccess) return hip_fail(e, "backward");
+    return PDS_OK;
+}
+
+}  // extern "C"
