@@ -19,7 +19,9 @@
  *  - ``stream`` is a hipStream_t (NULL = the null stream); every launch is
  *    stream-ordered on it and no call synchronises the host.
  *  - ``workspace`` is caller-owned device scratch of at least
- *    pds_workspace_bytes(...) bytes (it may be NULL when that size is 0).
+ *    pds_forward_workspace_bytes(...) / pds_backward_workspace_bytes(...)
+ *    bytes (it may be NULL when that size is 0).  Its contents need no
+ *    initialisation and are meaningless after the call.
  *  - Return value: PDS_OK (0) or a negative pds_status; no exception crosses
  *    the ABI.  pds_last_error() gives the message of the calling thread's
  *    last failure.
@@ -42,7 +44,7 @@ extern "C" {
 typedef enum pds_status {
     PDS_OK = 0,
     PDS_ERR_ARGUMENT = -1,    /* invalid sizes, null pointers, bad mode */
-    PDS_ERR_WORKSPACE = -2,   /* workspace smaller than pds_workspace_bytes() */
+    PDS_ERR_WORKSPACE = -2,   /* workspace smaller than the pds_*_workspace_bytes() size */
     PDS_ERR_HIP = -3,         /* a HIP runtime call or launch failed */
     PDS_ERR_CAPACITY = -4     /* a size beyond a documented maximum */
 } pds_status;
@@ -61,13 +63,19 @@ typedef struct pds_tuning {
     int32_t tile;          /* square screen tile edge in pixels: 8, 16, 32 or 64 */
     int32_t seg_photons;   /* photon records per tile segment (one workgroup each) */
     int32_t block_photons; /* photons per binning workgroup */
-    int32_t reserved[5];
+    int32_t capacity_pct;  /* records reserved per 100 photons (default 200); more
+                              records than that run the photon-order path */
+    int32_t reserved[4];
 } pds_tuning;
 
-/* Bytes of device workspace the forward or backward of this problem needs
- * in ``mode`` with tuning ``t`` (NULL = defaults). */
-size_t pds_workspace_bytes(int64_t N, int32_t C, int32_t H, int32_t W, int32_t max_pixel_radius,
-                           int32_t mode, const pds_tuning* t);
+/* Bytes of device workspace pds_forward_f32 needs for this problem in
+ * ``mode`` with tuning ``t`` (NULL = defaults); 0 if it needs none. */
+size_t pds_forward_workspace_bytes(int64_t N, int32_t C, int32_t H, int32_t W, int32_t max_pixel_radius,
+                                   int32_t mode, const pds_tuning* t);
+
+/* Bytes of device workspace pds_backward_f32 needs (0 if none). */
+size_t pds_backward_workspace_bytes(int64_t N, int32_t C, int32_t H, int32_t W, int32_t max_pixel_radius,
+                                    int32_t mode, const pds_tuning* t);
 
 /* Forward splat (reference pds_forward, PhotonDifferentialSplattig.cpp:12-25;
  * kernel photon_differentials.cu:60-113).  out [C,H,W] is overwritten. */

@@ -141,12 +141,15 @@ def photon_loss_f64(grad, c, E, x, M6, radius, cap) -> float:
         float(radius)))
 
 
-def central_fd(grad, photons, i, cap, h_x, h_m_rel=1e-4):
+def central_fd(grad, photons, i, cap, h_x, h_m_rel=1e-4, with_error=False):
     """Central finite differences of <g, F_i> for photon i.
 
     Returns ``(dx [2], dM [6])``: derivatives with respect to xp[:, i] (step
     ``h_x`` in normalised units) and Mp[:, :, i] (step ``h_m_rel`` times the
-    largest |M| entry of the photon), all in FP64.
+    largest |M| entry of the photon), all in FP64.  With ``with_error`` also
+    returns ``(ex [2], eM [6])`` = |FD(h) - FD(2h)|, an estimate of the
+    truncation error (large where a step crosses the kernel's support edge,
+    whose second derivative jumps).
     """
     E, xp, M, cp, radius = photons
     c = int(np.asarray(cp).reshape(-1)[i])
@@ -154,20 +157,31 @@ def central_fd(grad, photons, i, cap, h_x, h_m_rel=1e-4):
     x = np.asarray(xp, np.float64)[:, i].copy()
     m = np.asarray(M, np.float64)[:, :, i].reshape(6).copy()
     r = float(np.asarray(radius)[i])
-    dx = np.zeros(2)
-    for k in range(2):
-        xp_, xm_ = x.copy(), x.copy()
-        xp_[k] += h_x
-        xm_[k] -= h_x
-        dx[k] = (photon_loss_f64(grad, c, e, xp_, m, r, cap) - photon_loss_f64(grad, c, e, xm_, m, r, cap)) / (2 * h_x)
+
+    def loss(xv, mv):
+        return photon_loss_f64(grad, c, e, xv, mv, r, cap)
+
+    def fd(h_x_, h_m_):
+        dx = np.zeros(2)
+        for k in range(2):
+            xp_, xm_ = x.copy(), x.copy()
+            xp_[k] += h_x_
+            xm_[k] -= h_x_
+            dx[k] = (loss(xp_, m) - loss(xm_, m)) / (2 * h_x_)
+        dm = np.zeros(6)
+        for k in range(6):
+            mp_, mm_ = m.copy(), m.copy()
+            mp_[k] += h_m_
+            mm_[k] -= h_m_
+            dm[k] = (loss(x, mp_) - loss(x, mm_)) / (2 * h_m_)
+        return dx, dm
+
     hm = h_m_rel * float(np.abs(m).max())
-    dm = np.zeros(6)
-    for k in range(6):
-        mp_, mm_ = m.copy(), m.copy()
-        mp_[k] += hm
-        mm_[k] -= hm
-        dm[k] = (photon_loss_f64(grad, c, e, x, mp_, r, cap) - photon_loss_f64(grad, c, e, x, mm_, r, cap)) / (2 * hm)
-    return dx, dm
+    dx, dm = fd(h_x, hm)
+    if not with_error:
+        return dx, dm
+    dx2, dm2 = fd(2 * h_x, 2 * hm)
+    return dx, dm, np.abs(dx - dx2), np.abs(dm - dm2)
 
 
 def cpu_threads() -> int:

@@ -86,8 +86,8 @@ def _ptr(t: tp.Optional[torch.Tensor]):
     return ctypes.c_void_p(t.data_ptr()) if t is not None and t.numel() > 0 else ctypes.c_void_p(0)
 
 
-def _workspace(lib, N, C, H, W, cap, mode, tun, device):
-    nbytes = lib.pds_workspace_bytes(N, C, H, W, cap, mode, ctypes.byref(tun) if tun is not None else None)
+def _workspace(query, N, C, H, W, cap, mode, tun, device):
+    nbytes = query(N, C, H, W, cap, mode, ctypes.byref(tun) if tun is not None else None)
     if nbytes == 0:
         return None, 0
     return torch.empty(nbytes, dtype=torch.uint8, device=device), nbytes
@@ -107,7 +107,7 @@ def pds_forward(Ep: torch.Tensor, xp: torch.Tensor, Mp: torch.Tensor, cp: torch.
     lib = _native.lib()
     with torch.cuda.device(Ep.device):
         out = torch.empty(size, dtype=torch.float32, device=Ep.device)
-        ws, nbytes = _workspace(lib, N, C, H, W, int(max_pixel_radius), m, tun, Ep.device)
+        ws, nbytes = _workspace(lib.pds_forward_workspace_bytes, N, C, H, W, int(max_pixel_radius), m, tun, Ep.device)
         stream = torch.cuda.current_stream(Ep.device).cuda_stream
         rc = lib.pds_forward_f32(_ptr(Ep), _ptr(xp), _ptr(Mp), _ptr(cp), _ptr(radius), N, C, H, W,
                                  int(max_pixel_radius), m, ctypes.byref(tun) if tun is not None else None,
@@ -138,7 +138,8 @@ def pds_backward(grad_pds: torch.Tensor, Ep: torch.Tensor, xp: torch.Tensor, Mp:
         gE = torch.empty_like(Ep)
         gx = torch.empty_like(xp)
         gM = torch.empty_like(Mp)
-        ws, nbytes = _workspace(lib, N, C, H, W, int(max_pixel_radius), m, tun, Ep.device)
+        ws, nbytes = _workspace(lib.pds_backward_workspace_bytes, N, C, H, W, int(max_pixel_radius), m, tun,
+                                Ep.device)
         stream = torch.cuda.current_stream(Ep.device).cuda_stream
         rc = lib.pds_backward_f32(_ptr(grad_pds), _ptr(Ep), _ptr(xp), _ptr(Mp), _ptr(cp), _ptr(radius), N, C, H, W,
                                   int(max_pixel_radius), m, ctypes.byref(tun) if tun is not None else None,

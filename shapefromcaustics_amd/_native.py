@@ -27,7 +27,7 @@ class PdsTuning(ctypes.Structure):
     """Mirror of ``pds_tuning`` (include/pds.h)."""
 
     _fields_ = [("tile", ctypes.c_int32), ("seg_photons", ctypes.c_int32), ("block_photons", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 5)]
+                ("capacity_pct", ctypes.c_int32), ("reserved", ctypes.c_int32 * 4)]
 
 
 _lib = None
@@ -39,7 +39,8 @@ _sz = ctypes.c_size_t
 _tp = ctypes.POINTER(PdsTuning)
 
 SIGNATURES = {
-    "pds_workspace_bytes": ([_i64, _i32, _i32, _i32, _i32, _i32, _tp], _sz),
+    "pds_forward_workspace_bytes": ([_i64, _i32, _i32, _i32, _i32, _i32, _tp], _sz),
+    "pds_backward_workspace_bytes": ([_i64, _i32, _i32, _i32, _i32, _i32, _tp], _sz),
     "pds_forward_f32": ([_vp, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _i32, _i32, _i32, _tp, _vp, _vp, _sz, _vp], ctypes.c_int),
     "pds_backward_f32": ([_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _i32, _i32, _i32, _tp, _vp, _vp, _vp, _vp,
                           _sz, _vp], ctypes.c_int),

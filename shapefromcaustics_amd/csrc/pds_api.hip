@@ -79,6 +79,8 @@ pds::ImageArgs image_args(int32_t C, int32_t H, int32_t W, int32_t cap) {
     I.cap = cap;
     I.fw = (float)W;
     I.fh = (float)H;
+    I.inv_hw = (float)(2.0 / (double)W);
+    I.inv_hh = (float)(2.0 / (double)H);
     I.ell_trivial = ellipse_trivial(cap, W, H) ? 1 : 0;
     return I;
 }
@@ -91,13 +93,14 @@ const char* pds_last_error(void) { return g_last_error; }
 
 const char* pds_version(void) { return "pds_hip 0.1 gfx950"; }
 
-size_t pds_workspace_bytes(int64_t N, int32_t C, int32_t H, int32_t W, int32_t max_pixel_radius, int32_t mode,
-                           const pds_tuning* t) {
-    if (N < 0 || C <= 0 || H <= 0 || W <= 0) return 0;
-    if (mode == PDS_MODE_BINNED || mode == PDS_MODE_DETERMINISTIC || mode == PDS_MODE_AUTO) {
-        if (!ellipse_trivial(max_pixel_radius, W, H)) return 0;  // plain schedule, no workspace
-        return pds::binned_workspace_bytes(N, C, H, W, max_pixel_radius, mode, t);
-    }
+size_t pds_forward_workspace_bytes(int64_t N, int32_t C, int32_t H, int32_t W, int32_t max_pixel_radius,
+                                   int32_t mode, const pds_tuning* t) {
+    if (N < 0 || C <= 0 || H <= 0 || W <= 0 || mode == PDS_MODE_PLAIN) return 0;
+    return pds::binned_workspace_bytes(N, C, H, W, max_pixel_radius, mode, t,
+                                       ellipse_trivial(max_pixel_radius, W, H));
+}
+
+size_t pds_backward_workspace_bytes(int64_t, int32_t, int32_t, int32_t, int32_t, int32_t, const pds_tuning*) {
     return 0;
 }
 
@@ -110,7 +113,7 @@ int pds_forward_f32(const float* Ep, const float* xp, const float* Mp, const int
     const int64_t numel = (int64_t)C * H * W;
     if (numel == 0) return PDS_OK;
     if (!out) return fail(PDS_ERR_ARGUMENT, "null output pointer");
-    const size_t need = pds_workspace_bytes(N, C, H, W, max_pixel_radius, mode, t);
+    const size_t need = pds_forward_workspace_bytes(N, C, H, W, max_pixel_radius, mode, t);
     if (need > 0 && (workspace == nullptr || workspace_bytes < need))
         return fail(PDS_ERR_WORKSPACE, "workspace of %zu bytes is smaller than the %zu bytes required",
                     workspace ? workspace_bytes : (size_t)0, need);
@@ -118,7 +121,7 @@ int pds_forward_f32(const float* Ep, const float* xp, const float* Mp, const int
     const pds::PhotonArgs P = photon_args(Ep, xp, Mp, cp, radius, N);
     const pds::ImageArgs I = image_args(C, H, W, max_pixel_radius);
     hipError_t e;
-    if (mode == PDS_MODE_PLAIN || !I.ell_trivial) {
+    if (mode == PDS_MODE_PLAIN) {
         if ((e = pds::launch_fill_zero(out, numel, s)) != hipSuccess) return hip_fail(e, "zero fill");
         if ((e = pds::launch_forward_plain(P, I, out, s)) != hipSuccess) return hip_fail(e, "plain forward");
         return PDS_OK;

@@ -1,22 +1,802 @@
-// pds_binned.hip -- tile-binned forward (placeholder: photon-order flat path).
+// pds_binned.hip -- tile-binned forward splat for MI355X.
+//
+// The reference scatters every photon with global atomics
+// (PyOptix/kernel/photon_differentials.cu:60-113).  Global float atomics from
+// 256 CUs onto caustic hot spots serialise, so this forward instead sorts the
+// photons' (channel, screen tile) overlaps into bins and accumulates each tile
+// in LDS, flushing it to HBM with coalesced stores:
+//
+//   1. count     per-block LDS histogram of bin overlaps           -> hist[G][nbins]
+//   2. colsum    per-bin totals over blocks                         -> totals[nbins]
+//   3. plan      exclusive scan of totals, segment list (<= seg records each),
+//                capacity check, fixed-point scale for the deterministic mode
+//   4. offsets   exclusive per-(block, bin) record offsets, in place in hist
+//   5. scatter   each block re-walks its photons and writes one 32-byte record
+//                per overlap at slot = atomicAdd_LDS(cursor[bin], 1)
+//   6. zero      clears the tiles of bins split over several segments
+//   7. tile      one workgroup per segment: flattened walk of its records,
+//                LDS accumulation, flush (store, or atomic add when split)
+//   8. convert   deterministic mode: int64 fixed point -> float for split bins
+//
+// Atomic mode accumulates float in LDS (summation order varies run to run).
+// Deterministic mode accumulates every contribution as a 64-bit fixed-point
+// integer (quantum 2^e with 2^e * 2^61 >= N max(E) 3/pi, far below FP32
+// resolution of any pixel), so the sum is exact and order-independent: the
+// output is bitwise reproducible.  +inf contributions set bit 62.
+//
+// If the records exceed the workspace capacity, every binned kernel turns
+// into a no-op and the photon-order path (flattened walk, global atomics /
+// global int64 atomics) produces the result instead; a device flag chooses.
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
 #include "pds_binned.hpp"
 
 namespace pds {
 
-static thread_local const char* g_err = "";
+namespace {
+
+constexpr int kBinThreads = 512;
+constexpr int kMaxBins = 16384;
+constexpr int kMaxBinBlocks = 1024;
+constexpr int kColThreads = 1024;
+constexpr int kTileThreads = 256;
+constexpr int kTileChunk = 2;
+constexpr int kDefaultTile = 32;
+constexpr int kDefaultSeg = 2048;
+constexpr double kDefaultCapacity = 2.0;  // records per photon reserved in the workspace
+
+thread_local const char* g_err = "";
+
+struct Ctrl {
+    uint32_t nseg;        // segments planned
+    uint32_t overflow;    // 1: records exceed capacity -> photon-order path
+    uint32_t max_e_bits;  // max finite positive energy (float bits), deterministic mode
+    int32_t fx_exp;       // fixed-point quantum exponent e (quantum 2^e)
+    uint64_t total;       // records
+};
+
+struct Geom {
+    int32_t T, tshift, tiles_x, tiles_y, nbins, G, seg;
+    int64_t per_block, cap_records, max_segments;
+    bool det;
+};
+
+struct Layout {
+    size_t hist, totals, binoff, segs, ctrl, recs, acc64, end;
+};
+
+inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int ilog2(int v) {
+    int s = 0;
+    while ((1 << s) < v) ++s;
+    return s;
+}
+
+// Tile edge: the tuned one if given, else 32, raised until the bins fit the
+// LDS histogram.  0 means the binned path cannot run (photon order instead).
+bool make_geom(int64_t N, int32_t C, int32_t H, int32_t W, int32_t cap, int32_t mode, const pds_tuning* t, Geom& g) {
+    if (cap > 63) return false;  // packed record holds window radii < 64
+    int T = (t && t->tile > 0) ? t->tile : kDefaultTile;
+    if (T != 8 && T != 16 && T != 32 && T != 64) return false;
+    for (;;) {
+        const int64_t tx = (W + T - 1) / T, ty = (H + T - 1) / T;
+        if ((int64_t)C * tx * ty <= kMaxBins) break;
+        if (T >= 64) return false;
+        T *= 2;
+    }
+    g.T = T;
+    g.tshift = ilog2(T);
+    g.tiles_x = (W + T - 1) / T;
+    g.tiles_y = (H + T - 1) / T;
+    g.nbins = C * g.tiles_x * g.tiles_y;
+    g.seg = (t && t->seg_photons > 0) ? t->seg_photons : kDefaultSeg;
+    const int64_t bp = (t && t->block_photons > 0) ? t->block_photons : 0;
+    int64_t G = bp > 0 ? (N + bp - 1) / bp : (N + 4095) / 4096;
+    if (G > kMaxBinBlocks) G = kMaxBinBlocks;
+    if (G < 1) G = 1;
+    g.G = (int32_t)G;
+    g.per_block = (N + G - 1) / G;
+    double capf = kDefaultCapacity;
+    if (t && t->capacity_pct > 0) capf = t->capacity_pct / 100.0;
+    g.cap_records = (int64_t)(capf * (double)N) + 1024;
+    if (g.cap_records >= ((int64_t)1 << 32)) return false;  // 32-bit record offsets
+    g.max_segments = g.nbins + g.cap_records / g.seg + 1;
+    g.det = mode == PDS_MODE_DETERMINISTIC;
+    return true;
+}
+
+Layout make_layout(const Geom& g, int32_t C, int32_t H, int32_t W) {
+    Layout L;
+    size_t o = 0;
+    L.hist = o;
+    o = align256(o + sizeof(uint32_t) * (size_t)g.G * g.nbins);
+    L.totals = o;
+    o = align256(o + sizeof(uint32_t) * (size_t)g.nbins);
+    L.binoff = o;
+    o = align256(o + sizeof(uint32_t) * (size_t)g.nbins);
+    L.segs = o;
+    o = align256(o + sizeof(int4) * (size_t)g.max_segments);
+    L.ctrl = o;
+    o = align256(o + sizeof(Ctrl));
+    L.recs = o;
+    o = align256(o + 32 * (size_t)g.cap_records);
+    L.acc64 = o;
+    if (g.det) o = align256(o + sizeof(uint64_t) * (size_t)C * H * W);
+    L.end = o;
+    return L;
+}
+
+// ---------------------------------------------------------------------------
+// photon -> clipped footprint box (forward semantics)
+// ---------------------------------------------------------------------------
+struct BinPhoton {
+    bool ok;
+    int32_t c;
+    int32_t x0, x1, y0, y1;  // box on the image, inclusive
+    Frame f;
+    float e;
+    int32_t rx, ry, ox, oy;  // window radii and pxc - ix + 1, pyc - iy + 1
+};
+
+// Photons that can add to the image: channel in range, E > 0 (the reference's
+// val > 0 filter, photon_differentials.cu:106), finite frame.  Their box is
+// window (:80-96) n image n footprint bounding box.
+__device__ __forceinline__ BinPhoton bin_photon(const PhotonArgs& P, const ImageArgs& I, int64_t i) {
+    BinPhoton b;
+    b.ok = false;
+    const int64_t c = P.cp[i];
+    if (c < 0 || c >= I.C) return b;
+    const int64_t N = P.N;
+    const float e = P.E[i];
+    if (!(e > 0.0f)) return b;
+    const float cx = P.xp[i], cy = P.xp[N + i];
+    const Window win = photon_window(P.radius[i], cx, cy, I.W, I.H, I.cap);
+    if (win.rx < 0 || win.ry < 0) return b;
+    int64_t x0 = max((int64_t)win.pxc - win.rx, (int64_t)0);
+    int64_t x1 = min((int64_t)win.pxc + win.rx, (int64_t)I.W - 1);
+    int64_t y0 = max((int64_t)win.pyc - win.ry, (int64_t)0);
+    int64_t y1 = min((int64_t)win.pyc + win.ry, (int64_t)I.H - 1);
+    if (x0 > x1 || y0 > y1) return b;
+    const float m00 = P.M[i], m01 = P.M[N + i], m10 = P.M[3 * N + i], m11 = P.M[4 * N + i];
+    const float poison = P.M[2 * N + i] * 0.0f + P.M[5 * N + i] * 0.0f;
+    // a non-finite centre, matrix or M[:,2] makes q NaN or infinite: K = 0, no contribution
+    if (!(isfinite(cx) && isfinite(cy) && isfinite(m00) && isfinite(m01) && isfinite(m10) && isfinite(m11) &&
+          poison == 0.0f))
+        return b;
+    b.f = photon_frame(cx, cy, m00, m01, m10, m11, 0.0f, I.W, I.H, I.inv_hw, I.inv_hh);
+    {
+        int32_t a0 = (int32_t)x0, a1 = (int32_t)x1, c0 = (int32_t)y0, c1 = (int32_t)y1;
+        clip_to_ellipse(b.f, a0, a1, c0, c1);
+        if (a0 > a1 || c0 > c1) return b;
+        x0 = a0;
+        x1 = a1;
+        y0 = c0;
+        y1 = c1;
+    }
+    b.x0 = (int32_t)x0;
+    b.x1 = (int32_t)x1;
+    b.y0 = (int32_t)y0;
+    b.y1 = (int32_t)y1;
+    b.c = (int32_t)c;
+    b.e = e;
+    b.rx = win.rx;
+    b.ry = win.ry;
+    b.ox = win.pxc - b.f.ix + 1;  // in {0, 1, 2}: trunc vs round of the same double
+    b.oy = win.pyc - b.f.iy + 1;
+    b.ok = true;
+    return b;
+}
+
+// ---------------------------------------------------------------------------
+// 32-byte record: (fx, fy, e, packed), (a00, a01, a10, a11)
+// packed: ix - tile centre (int8) | iy - tile centre (int8) << 8 | rx << 16 |
+//         ry << 22 | ox << 28 | oy << 30
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t pack_rec(int32_t dcx, int32_t dcy, int32_t rx, int32_t ry, int32_t ox, int32_t oy) {
+    return ((uint32_t)(dcx & 0xff)) | ((uint32_t)(dcy & 0xff) << 8) | ((uint32_t)rx << 16) | ((uint32_t)ry << 22) |
+           ((uint32_t)ox << 28) | ((uint32_t)oy << 30);
+}
+
+// ---------------------------------------------------------------------------
+// 1. count
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBinThreads) void bin_count_kernel(PhotonArgs P, ImageArgs I, Geom g,
+                                                                uint32_t* __restrict__ hist, Ctrl* ctrl) {
+    extern __shared__ uint32_t lh[];  // nbins entries
+    __shared__ float wmax[kBinThreads / 64];
+    for (int k = threadIdx.x; k < g.nbins; k += kBinThreads) lh[k] = 0;
+    __syncthreads();
+    const int64_t lo = (int64_t)blockIdx.x * g.per_block;
+    const int64_t hi = min(lo + g.per_block, P.N);
+    float emax = 0.0f;
+    for (int64_t i = lo + threadIdx.x; i < hi; i += kBinThreads) {
+        const BinPhoton b = bin_photon(P, I, i);
+        if (!b.ok) continue;
+        if (isfinite(b.e)) emax = fmaxf(emax, b.e);
+        const int tx0 = b.x0 >> g.tshift, tx1 = b.x1 >> g.tshift;
+        const int ty0 = b.y0 >> g.tshift, ty1 = b.y1 >> g.tshift;
+        for (int ty = ty0; ty <= ty1; ++ty)
+            for (int tx = tx0; tx <= tx1; ++tx) atomicAdd(&lh[(b.c * g.tiles_y + ty) * g.tiles_x + tx], 1u);
+    }
+    if (g.det) {
+        for (int o = 32; o > 0; o >>= 1) emax = fmaxf(emax, __shfl_xor(emax, o));
+        if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = emax;
+    }
+    __syncthreads();
+    uint32_t* row = hist + (size_t)blockIdx.x * g.nbins;
+    for (int k = threadIdx.x; k < g.nbins; k += kBinThreads) row[k] = lh[k];
+    if (g.det && threadIdx.x == 0) {
+        float m = 0.0f;
+        for (int w = 0; w < kBinThreads / 64; ++w) m = fmaxf(m, wmax[w]);
+        atomicMax(&ctrl->max_e_bits, __float_as_uint(m));  // positive floats order like their bits
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 2. per-bin totals over blocks (and 4. offsets): 64 bins per workgroup,
+//    16 wavefronts each own a contiguous range of block rows.
+// ---------------------------------------------------------------------------
+template <bool OFFSETS>
+__global__ __launch_bounds__(kColThreads) void bin_columns_kernel(Geom g, uint32_t* __restrict__ hist,
+                                                                  uint32_t* __restrict__ totals,
+                                                                  const uint32_t* __restrict__ binoff,
+                                                                  const Ctrl* ctrl) {
+    constexpr int WAVES = kColThreads / 64;
+    __shared__ uint32_t part[WAVES][64];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int bin = blockIdx.x * 64 + lane;
+    const int rows = (g.G + WAVES - 1) / WAVES;
+    const int r0 = w * rows, r1 = min(r0 + rows, g.G);
+    uint32_t s = 0;
+    if (bin < g.nbins)
+        for (int r = r0; r < r1; ++r) s += hist[(size_t)r * g.nbins + bin];
+    part[w][lane] = s;
+    __syncthreads();
+    if (!OFFSETS) {
+        if (w == 0 && bin < g.nbins) {
+            uint32_t t = 0;
+            for (int k = 0; k < WAVES; ++k) t += part[k][lane];
+            totals[bin] = t;
+        }
+        return;
+    }
+    if (ctrl->overflow || bin >= g.nbins) return;
+    uint32_t run = binoff[bin];
+    for (int k = 0; k < w; ++k) run += part[k][lane];
+    for (int r = r0; r < r1; ++r) {
+        uint32_t* p = hist + (size_t)r * g.nbins + bin;
+        const uint32_t v = *p;
+        *p = run;
+        run += v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 3. plan: one workgroup
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void bin_plan_kernel(Geom g, int64_t N, const uint32_t* __restrict__ totals,
+                                                        uint32_t* __restrict__ binoff, int4* __restrict__ segs,
+                                                        Ctrl* ctrl) {
+    __shared__ uint64_t wsum[16];
+    __shared__ uint32_t wseg[16];
+    __shared__ uint64_t base_rec;
+    __shared__ uint32_t base_seg;
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    if (tid == 0) {
+        base_rec = 0;
+        base_seg = 0;
+    }
+    __syncthreads();
+    // bins in chunks of 1024, one per thread, carried across chunks
+    for (int c0 = 0; c0 < g.nbins; c0 += 1024) {
+        const int bin = c0 + tid;
+        const uint32_t t = bin < g.nbins ? totals[bin] : 0u;
+        const uint32_t ns = bin < g.nbins ? max(1u, (t + g.seg - 1) / g.seg) : 0u;
+        // inclusive wave scans (64-bit records, 32-bit segments)
+        uint64_t a = t;
+        uint32_t b = ns;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint64_t ya = __shfl_up(a, o);
+            const uint32_t yb = __shfl_up(b, o);
+            if (lane >= o) {
+                a += ya;
+                b += yb;
+            }
+        }
+        if (lane == 63) {
+            wsum[w] = a;
+            wseg[w] = b;
+        }
+        __syncthreads();
+        uint64_t pa = base_rec;
+        uint32_t pb = base_seg;
+        for (int k = 0; k < w; ++k) {
+            pa += wsum[k];
+            pb += wseg[k];
+        }
+        const uint64_t off = pa + a - t;
+        const uint32_t soff = pb + b - ns;
+        if (bin < g.nbins) {
+            binoff[bin] = (uint32_t)off;
+            const bool fits = off + t <= (uint64_t)g.cap_records && soff + ns <= (uint64_t)g.max_segments;
+            if (fits)
+                for (uint32_t s = 0; s < ns; ++s) {
+                    const uint32_t st = (uint32_t)off + s * g.seg;
+                    segs[soff + s] = make_int4(bin, (int)st, (int)min((uint32_t)g.seg, t - s * g.seg),
+                                               (ns > 1 ? 1 : 0) | (s == 0 ? 2 : 0));
+                }
+        }
+        __syncthreads();
+        if (tid == 1023) {
+            base_rec = pa + a;
+            base_seg = pb + b;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        ctrl->total = base_rec;
+        const bool over = base_rec > (uint64_t)g.cap_records || base_seg > (uint64_t)g.max_segments;
+        ctrl->overflow = over ? 1u : 0u;
+        ctrl->nseg = over ? 0u : base_seg;
+        // fixed-point quantum: 2^e with N * max(E) * 3/pi <= 2^(e + 61)
+        const float m = __uint_as_float(ctrl->max_e_bits);
+        int e = 0;
+        if (m > 0.0f) {
+            const double bound = (double)N * (double)m * 0.9549296585513720;
+            e = (int)ceil(log2(bound)) - 61;
+        }
+        ctrl->fx_exp = e;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 5. scatter
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBinThreads) void bin_scatter_kernel(PhotonArgs P, ImageArgs I, Geom g,
+                                                                  const uint32_t* __restrict__ hist,
+                                                                  uint4* __restrict__ recs, const Ctrl* ctrl) {
+    extern __shared__ uint32_t cur[];  // nbins entries
+    if (ctrl->overflow) return;
+    const uint32_t* row = hist + (size_t)blockIdx.x * g.nbins;
+    for (int k = threadIdx.x; k < g.nbins; k += kBinThreads) cur[k] = row[k];
+    __syncthreads();
+    const int64_t lo = (int64_t)blockIdx.x * g.per_block;
+    const int64_t hi = min(lo + g.per_block, P.N);
+    const int half = g.T >> 1;
+    for (int64_t i = lo + threadIdx.x; i < hi; i += kBinThreads) {
+        const BinPhoton b = bin_photon(P, I, i);
+        if (!b.ok) continue;
+        const uint4 r0 = make_uint4(__float_as_uint(b.f.fx), __float_as_uint(b.f.fy), __float_as_uint(b.e), 0u);
+        const uint4 r1 = make_uint4(__float_as_uint(b.f.a00), __float_as_uint(b.f.a01), __float_as_uint(b.f.a10),
+                                    __float_as_uint(b.f.a11));
+        const int tx0 = b.x0 >> g.tshift, tx1 = b.x1 >> g.tshift;
+        const int ty0 = b.y0 >> g.tshift, ty1 = b.y1 >> g.tshift;
+        for (int ty = ty0; ty <= ty1; ++ty) {
+            for (int tx = tx0; tx <= tx1; ++tx) {
+                const uint32_t slot = atomicAdd(&cur[(b.c * g.tiles_y + ty) * g.tiles_x + tx], 1u);
+                uint4 q = r0;
+                q.w = pack_rec(b.f.ix - ((tx << g.tshift) + half), b.f.iy - ((ty << g.tshift) + half), b.rx, b.ry,
+                               b.ox, b.oy);
+                recs[2 * (size_t)slot] = q;
+                recs[2 * (size_t)slot + 1] = r1;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 6. zero the tiles of split bins (the output in atomic mode, the int64
+//    accumulator in deterministic mode)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kTileThreads) void bin_zero_split_kernel(Geom g, ImageArgs I, const int4* __restrict__ segs,
+                                                                      const Ctrl* ctrl, T* __restrict__ dst) {
+    const uint32_t s = blockIdx.x;
+    if (s >= ctrl->nseg) return;
+    const int4 d = segs[s];
+    if ((d.w & 3) != 3) return;  // first segment of a split bin only
+    const int tpc = g.tiles_x * g.tiles_y;
+    const int c = d.x / tpc, t = d.x - c * tpc;
+    const int ty = t / g.tiles_x, tx = t - ty * g.tiles_x;
+    const int X0 = tx << g.tshift, Y0 = ty << g.tshift;
+    for (int k = threadIdx.x; k < g.T * g.T; k += kTileThreads) {
+        const int px = X0 + (k & (g.T - 1)), py = Y0 + (k >> g.tshift);
+        if (px < I.W && py < I.H) dst[((int64_t)c * I.H + py) * I.W + px] = T(0);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 7. tile
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned long long to_fixed(float val, int e) {
+    // val in (0, 2^(e+61)]: quantise to the 2^e grid; +inf sets bit 62
+    if (!(val < INFINITY)) return 1ull << 62;
+    return (unsigned long long)__float2ll_rn(ldexpf(val, -e));
+}
+
+__device__ __forceinline__ float from_fixed(unsigned long long v, int e) {
+    if (v & (1ull << 62)) return INFINITY;
+    return (float)ldexp((double)(long long)v, e);
+}
+
+template <bool DET>
+__global__ __launch_bounds__(kTileThreads) void bin_tile_kernel(Geom g, ImageArgs I, const int4* __restrict__ segs,
+                                                                const uint4* __restrict__ recs, const Ctrl* ctrl,
+                                                                float* __restrict__ out,
+                                                                unsigned long long* __restrict__ acc64) {
+    using Acc = typename std::conditional<DET, unsigned long long, float>::type;
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ WaveRecs wr[kTileThreads / 64];
+    Acc* tile = reinterpret_cast<Acc*>(smem);
+    const uint32_t s = blockIdx.x;
+    if (s >= ctrl->nseg) return;
+    const int4 d = segs[s];
+    const int fe = DET ? ctrl->fx_exp : 0;
+    const int T = g.T, TT = T * T;
+    for (int k = threadIdx.x; k < TT; k += kTileThreads) tile[k] = Acc(0);
+    const int tpc = g.tiles_x * g.tiles_y;
+    const int c = d.x / tpc, tt = d.x - c * tpc;
+    const int ty = tt / g.tiles_x, tx = tt - ty * g.tiles_x;
+    const int X0 = tx << g.tshift, Y0 = ty << g.tshift;
+    const int X1 = min(X0 + T, I.W) - 1, Y1 = min(Y0 + T, I.H) - 1;
+    const int half = T >> 1;
+    __syncthreads();
+
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    WaveRecs& R = wr[w];
+    R.mark[lane] = -1;
+    for (int b0 = w * 64; b0 < d.z; b0 += kTileThreads) {
+        FlatRec r;
+        r.nitems = 0;
+        r.kx0 = r.ky0 = r.bw = r.base = 0;
+        r.cpr = 1;
+        r.inv_cpr = 1.0f;
+        r.e = 0.f;
+        r.f.fx = r.f.fy = r.f.a00 = r.f.a01 = r.f.a10 = r.f.a11 = 0.f;
+        if (b0 + lane < d.z) {
+            const size_t j = (size_t)d.y + b0 + lane;
+            const uint4 q0 = recs[2 * j], q1 = recs[2 * j + 1];
+            const uint32_t pk = q0.w;
+            const int ix = X0 + half + (int)(int8_t)(pk & 0xff);
+            const int iy = Y0 + half + (int)(int8_t)((pk >> 8) & 0xff);
+            const int rx = (pk >> 16) & 63, ry = (pk >> 22) & 63;
+            const int pxc = ix + (int)((pk >> 28) & 3) - 1, pyc = iy + (int)(pk >> 30) - 1;
+            r.f.fx = __uint_as_float(q0.x);
+            r.f.fy = __uint_as_float(q0.y);
+            r.e = __uint_as_float(q0.z);
+            r.f.a00 = __uint_as_float(q1.x);
+            r.f.a01 = __uint_as_float(q1.y);
+            r.f.a10 = __uint_as_float(q1.z);
+            r.f.a11 = __uint_as_float(q1.w);
+            int x0 = max(max(pxc - rx, X0), 0), x1 = min(pxc + rx, X1);
+            int y0 = max(max(pyc - ry, Y0), 0), y1 = min(pyc + ry, Y1);
+            r.f.ix = ix;
+            r.f.iy = iy;
+            clip_to_ellipse(r.f, x0, x1, y0, y1);
+            if (x0 <= x1 && y0 <= y1) {
+                const int bw = x1 - x0 + 1, bh = y1 - y0 + 1;
+                r.cpr = (bw + kTileChunk - 1) / kTileChunk;
+                r.inv_cpr = 1.0f / (float)r.cpr;
+                r.nitems = bh * r.cpr;
+                r.kx0 = x0 - ix;
+                r.ky0 = y0 - iy;
+                r.bw = bw;
+                r.base = (y0 - Y0) * T + (x0 - X0);  // index into the LDS tile
+            }
+        }
+        const int64_t total = publish_recs(R, r, lane);
+        ImageArgs TI = I;
+        TI.W = T;  // walk_items computes base + row * W + col: LDS tile row pitch
+        walk_items<kTileChunk>(R, r, TI, lane, total, [&](const FlatItem& it) {
+#pragma unroll
+            for (int k = 0; k < kTileChunk; ++k) {
+                if (k < it.n) {
+                    const float dx = (float)(it.kx + k) - it.f.fx;
+                    const float u = it.f.a00 * dx + it.f.a01 * it.dy;
+                    const float v = it.f.a10 * dx + it.f.a11 * it.dy;
+                    const float val = silverman(u * u + v * v) * it.e;
+                    if (val > 0.0f) {
+                        if constexpr (DET) {
+                            const unsigned long long fxv = to_fixed(val, fe);
+                            if (fxv >> 62)
+                                atomicOr(&tile[it.pix + k], fxv);
+                            else
+                                atomicAdd(&tile[it.pix + k], fxv);
+                        } else {
+                            atomicAdd(&tile[it.pix + k], val);
+                        }
+                    }
+                }
+            }
+        });
+    }
+    __syncthreads();
+    const bool split = (d.w & 1) != 0;
+    for (int k = threadIdx.x; k < TT; k += kTileThreads) {
+        const int px = X0 + (k & (T - 1)), py = Y0 + (k >> g.tshift);
+        if (px >= I.W || py >= I.H) continue;
+        const int64_t o = ((int64_t)c * I.H + py) * I.W + px;
+        const Acc v = tile[k];
+        if constexpr (DET) {
+            if (!split)
+                out[o] = from_fixed(v, fe);
+            else if (v != 0)
+                atomicAdd(&acc64[o], v);  // integer addition: order-independent
+        } else {
+            if (!split)
+                out[o] = v;
+            else if (v != 0.0f)
+                unsafeAtomicAdd(&out[o], v);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 8. convert split tiles (deterministic mode)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kTileThreads) void bin_convert_split_kernel(Geom g, ImageArgs I,
+                                                                         const int4* __restrict__ segs,
+                                                                         const Ctrl* ctrl,
+                                                                         const unsigned long long* __restrict__ acc64,
+                                                                         float* __restrict__ out) {
+    const uint32_t s = blockIdx.x;
+    if (s >= ctrl->nseg) return;
+    const int4 d = segs[s];
+    if ((d.w & 3) != 3) return;
+    const int tpc = g.tiles_x * g.tiles_y;
+    const int c = d.x / tpc, t = d.x - c * tpc;
+    const int ty = t / g.tiles_x, tx = t - ty * g.tiles_x;
+    const int X0 = tx << g.tshift, Y0 = ty << g.tshift;
+    const int fe = ctrl->fx_exp;
+    for (int k = threadIdx.x; k < g.T * g.T; k += kTileThreads) {
+        const int px = X0 + (k & (g.T - 1)), py = Y0 + (k >> g.tshift);
+        if (px < I.W && py < I.H) {
+            const int64_t o = ((int64_t)c * I.H + py) * I.W + px;
+            out[o] = from_fixed(acc64[o], fe);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// photon-order fallback when the records exceed the workspace capacity
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void guarded_fill_kernel(const Ctrl* ctrl, float* __restrict__ out, int64_t n) {
+    if (!ctrl->overflow) return;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) out[i] = 0.0f;
+}
+
+__global__ __launch_bounds__(256) void guarded_fill64_kernel(const Ctrl* ctrl, unsigned long long* __restrict__ p,
+                                                             int64_t n) {
+    if (!ctrl->overflow) return;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0ull;
+}
+
+__global__ __launch_bounds__(256) void guarded_convert_kernel(const Ctrl* ctrl, const unsigned long long* __restrict__ p,
+                                                              float* __restrict__ out, int64_t n) {
+    if (!ctrl->overflow) return;
+    const int fe = ctrl->fx_exp;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) out[i] = from_fixed(p[i], fe);
+}
+
+template <bool DET>
+__global__ __launch_bounds__(256) void guarded_flat_kernel(PhotonArgs P, ImageArgs I, const Ctrl* ctrl,
+                                                           float* __restrict__ out,
+                                                           unsigned long long* __restrict__ acc64) {
+    __shared__ WaveRecs recs[4];
+    if (!ctrl->overflow) return;
+    const int fe = DET ? ctrl->fx_exp : 0;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    WaveRecs& R = recs[w];
+    R.mark[lane] = -1;
+    const int64_t nbatch = (P.N + 63) >> 6;
+    for (int64_t b = (int64_t)blockIdx.x * 4 + w; b < nbatch; b += (int64_t)gridDim.x * 4) {
+        const int64_t i = (b << 6) + lane;
+        FlatRec r;
+        setup_flat_rec<2>(P, I, i, /*forward=*/true, r);
+        const int64_t total = publish_recs(R, r, lane);
+        walk_items<2>(R, r, I, lane, total, [&](const FlatItem& it) {
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                if (k < it.n) {
+                    const float dx = (float)(it.kx + k) - it.f.fx;
+                    const float u = it.f.a00 * dx + it.f.a01 * it.dy;
+                    const float v = it.f.a10 * dx + it.f.a11 * it.dy;
+                    const float val = silverman(u * u + v * v) * it.e;
+                    if (val > 0.0f) {
+                        if constexpr (DET) {
+                            const unsigned long long fxv = to_fixed(val, fe);
+                            if (fxv >> 62)
+                                atomicOr(&acc64[it.pix + k], fxv);
+                            else
+                                atomicAdd(&acc64[it.pix + k], fxv);
+                        } else {
+                            unsafeAtomicAdd(&out[it.pix + k], val);
+                        }
+                    }
+                }
+            }
+        });
+    }
+}
+
+// maximum finite positive energy of the photons that can contribute
+__global__ __launch_bounds__(256) void emax_kernel(PhotonArgs P, ImageArgs I, Ctrl* ctrl) {
+    __shared__ float wmax[4];
+    float m = 0.0f;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < P.N; i += stride) {
+        const int64_t c = P.cp[i];
+        const float e = P.E[i];
+        if (c >= 0 && c < I.C && e > 0.0f && isfinite(e)) m = fmaxf(m, e);
+    }
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicMax(&ctrl->max_e_bits, __float_as_uint(fmaxf(fmaxf(wmax[0], wmax[1]),
+                                                                               fmaxf(wmax[2], wmax[3]))));
+}
+
+// fixed-point quantum for the photon-order deterministic path (which the
+// guarded kernels run: overflow = 1)
+__global__ void fixed_point_kernel(Ctrl* ctrl, int64_t N) {
+    const float m = __uint_as_float(ctrl->max_e_bits);
+    ctrl->fx_exp = m > 0.0f ? (int)ceil(log2((double)N * (double)m * 0.9549296585513720)) - 61 : 0;
+    ctrl->overflow = 1;
+}
+
+// Deterministic forward on images where the reference's window ellipse test
+// is active (photon_differentials.cu:93): the plain schedule, int64 atomics.
+__global__ __launch_bounds__(256) void fwd_plain_fixed_kernel(PhotonArgs P, ImageArgs I, const Ctrl* ctrl,
+                                                              unsigned long long* __restrict__ acc64) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.N) return;
+    const PlainSetup s = plain_setup(P, I, i);
+    if (!s.ok) return;
+    const int fe = ctrl->fx_exp;
+    unsigned long long* plane = acc64 + (int64_t)s.c * I.H * I.W;
+    const Frame& f = s.f;
+    for (int64_t py = s.y0; py <= s.y1; ++py) {
+        const float dy = (float)((int32_t)py - f.iy) - f.fy;
+        for (int64_t px = s.x0; px <= s.x1; ++px) {
+            if (!I.ell_trivial && !window_ellipse_ok((int32_t)(px - s.win.pxc), (int32_t)(py - s.win.pyc), I.W, I.H))
+                continue;
+            const float dx = (float)((int32_t)px - f.ix) - f.fx;
+            const float u = f.a00 * dx + f.a01 * dy;
+            const float v = f.a10 * dx + f.a11 * dy;
+            const float val = silverman(u * u + v * v) * s.e;
+            if (val > 0.0f) {
+                const unsigned long long q = to_fixed(val, fe);
+                if (q >> 62)
+                    atomicOr(&plane[py * I.W + px], q);
+                else
+                    atomicAdd(&plane[py * I.W + px], q);
+            }
+        }
+    }
+}
+
+__global__ void ctrl_init_kernel(Ctrl* ctrl) {
+    ctrl->nseg = 0;
+    ctrl->overflow = 0;
+    ctrl->max_e_bits = 0;
+    ctrl->fx_exp = 0;
+    ctrl->total = 0;
+}
+
+}  // namespace
 
 const char* binned_last_error() { return g_err; }
 
-size_t binned_workspace_bytes(int64_t, int32_t, int32_t, int32_t, int32_t, int32_t, const pds_tuning*) { return 0; }
-
-int binned_forward(const PhotonArgs& P, const ImageArgs& I, int32_t mode, const pds_tuning*, float* out, void*,
-                   size_t, hipStream_t s) {
-    if (mode != PDS_MODE_AUTO) {
-        g_err = "binned modes are not built yet";
-        return PDS_ERR_ARGUMENT;
+size_t binned_workspace_bytes(int64_t N, int32_t C, int32_t H, int32_t W, int32_t cap, int32_t mode,
+                              const pds_tuning* t, bool ell_trivial) {
+    if (mode == PDS_MODE_PLAIN) return 0;
+    const bool det = mode == PDS_MODE_DETERMINISTIC;
+    Geom g;
+    if (!ell_trivial || !make_geom(N, C, H, W, cap, mode, t, g)) {
+        // photon-order paths: only the deterministic one needs the int64 image and a control block
+        if (!det) return 0;
+        return align256(sizeof(Ctrl)) + sizeof(uint64_t) * (size_t)C * H * W;
     }
-    if (launch_fill_zero(out, (int64_t)I.C * I.H * I.W, s) != hipSuccess) return PDS_ERR_HIP;
-    if (launch_forward_flat(P, I, out, s) != hipSuccess) return PDS_ERR_HIP;
+    return make_layout(g, C, H, W).end;
+}
+
+int binned_forward(const PhotonArgs& P, const ImageArgs& I, int32_t mode, const pds_tuning* t, float* out, void* ws,
+                   size_t ws_bytes, hipStream_t s) {
+    const int64_t numel = (int64_t)I.C * I.H * I.W;
+    const bool det = mode == PDS_MODE_DETERMINISTIC;
+    Geom g;
+    if (!I.ell_trivial || !make_geom(P.N, I.C, I.H, I.W, I.cap, mode, t, g)) {
+        // no binning possible (tiny image with the window ellipse test active,
+        // cap > 63, or too many bins): photon-order paths
+        if (!det) {
+            if (launch_fill_zero(out, numel, s) != hipSuccess) return PDS_ERR_HIP;
+            if (!I.ell_trivial) return launch_forward_plain(P, I, out, s) == hipSuccess ? PDS_OK : PDS_ERR_HIP;
+            if (launch_forward_flat(P, I, out, s) != hipSuccess) return PDS_ERR_HIP;
+            return PDS_OK;
+        }
+        // deterministic: int64 fixed point over the whole image, photon order
+        if (ws == nullptr || ws_bytes < align256(sizeof(Ctrl)) + sizeof(uint64_t) * (size_t)numel) {
+            g_err = "workspace too small";
+            return PDS_ERR_WORKSPACE;
+        }
+        unsigned char* base = (unsigned char*)ws;
+        Ctrl* ctrl = (Ctrl*)base;
+        unsigned long long* acc = (unsigned long long*)(base + align256(sizeof(Ctrl)));
+        hipLaunchKernelGGL(ctrl_init_kernel, dim3(1), dim3(1), 0, s, ctrl);
+        hipLaunchKernelGGL(emax_kernel, dim3(1024), dim3(256), 0, s, P, I, ctrl);
+        hipLaunchKernelGGL(fixed_point_kernel, dim3(1), dim3(1), 0, s, ctrl, P.N);
+        hipLaunchKernelGGL(guarded_fill64_kernel, dim3(1024), dim3(256), 0, s, (const Ctrl*)ctrl, acc, numel);
+        if (I.ell_trivial)
+            hipLaunchKernelGGL(guarded_flat_kernel<true>, dim3(1024), dim3(256), 0, s, P, I, (const Ctrl*)ctrl, out,
+                               acc);
+        else if (P.N > 0)
+            hipLaunchKernelGGL(fwd_plain_fixed_kernel, dim3((unsigned)((P.N + 255) / 256)), dim3(256), 0, s, P, I,
+                               (const Ctrl*)ctrl, acc);
+        hipLaunchKernelGGL(guarded_convert_kernel, dim3(1024), dim3(256), 0, s, (const Ctrl*)ctrl,
+                           (const unsigned long long*)acc, out, numel);
+        return hipGetLastError() == hipSuccess ? PDS_OK : PDS_ERR_HIP;
+    }
+    const Layout L = make_layout(g, I.C, I.H, I.W);
+    if (ws_bytes < L.end || ws == nullptr) {
+        g_err = "workspace too small";
+        return PDS_ERR_WORKSPACE;
+    }
+    unsigned char* base = (unsigned char*)ws;
+    uint32_t* hist = (uint32_t*)(base + L.hist);
+    uint32_t* totals = (uint32_t*)(base + L.totals);
+    uint32_t* binoff = (uint32_t*)(base + L.binoff);
+    int4* segs = (int4*)(base + L.segs);
+    Ctrl* ctrl = (Ctrl*)(base + L.ctrl);
+    uint4* recs = (uint4*)(base + L.recs);
+    unsigned long long* acc64 = det ? (unsigned long long*)(base + L.acc64) : nullptr;
+    const unsigned colblocks = (unsigned)((g.nbins + 63) / 64);
+
+    hipLaunchKernelGGL(ctrl_init_kernel, dim3(1), dim3(1), 0, s, ctrl);
+    const size_t hist_lds = sizeof(uint32_t) * (size_t)g.nbins;
+    hipLaunchKernelGGL(bin_count_kernel, dim3(g.G), dim3(kBinThreads), hist_lds, s, P, I, g, hist, ctrl);
+    hipLaunchKernelGGL(bin_columns_kernel<false>, dim3(colblocks), dim3(kColThreads), 0, s, g, hist, totals,
+                       (const uint32_t*)binoff, (const Ctrl*)ctrl);
+    hipLaunchKernelGGL(bin_plan_kernel, dim3(1), dim3(1024), 0, s, g, P.N, (const uint32_t*)totals, binoff, segs,
+                       ctrl);
+    hipLaunchKernelGGL(bin_columns_kernel<true>, dim3(colblocks), dim3(kColThreads), 0, s, g, hist, totals,
+                       (const uint32_t*)binoff, (const Ctrl*)ctrl);
+    hipLaunchKernelGGL(bin_scatter_kernel, dim3(g.G), dim3(kBinThreads), hist_lds, s, P, I, g, (const uint32_t*)hist, recs,
+                       (const Ctrl*)ctrl);
+    const unsigned nsegs = (unsigned)g.max_segments;
+    const size_t tile_lds = (size_t)g.T * g.T * (det ? 8 : 4);
+    if (det) {
+        hipLaunchKernelGGL(bin_zero_split_kernel<unsigned long long>, dim3(nsegs), dim3(kTileThreads), 0, s, g, I,
+                           (const int4*)segs, (const Ctrl*)ctrl, acc64);
+        hipLaunchKernelGGL(bin_tile_kernel<true>, dim3(nsegs), dim3(kTileThreads), tile_lds, s, g, I,
+                           (const int4*)segs, (const uint4*)recs, (const Ctrl*)ctrl, out, acc64);
+        hipLaunchKernelGGL(bin_convert_split_kernel, dim3(nsegs), dim3(kTileThreads), 0, s, g, I, (const int4*)segs,
+                           (const Ctrl*)ctrl, (const unsigned long long*)acc64, out);
+    } else {
+        hipLaunchKernelGGL(bin_zero_split_kernel<float>, dim3(nsegs), dim3(kTileThreads), 0, s, g, I,
+                           (const int4*)segs, (const Ctrl*)ctrl, out);
+        hipLaunchKernelGGL(bin_tile_kernel<false>, dim3(nsegs), dim3(kTileThreads), tile_lds, s, g, I,
+                           (const int4*)segs, (const uint4*)recs, (const Ctrl*)ctrl, out,
+                           (unsigned long long*)nullptr);
+    }
+    // capacity overflow: the photon-order path, enabled by the device flag
+    const unsigned fb = 1024;
+    if (det) {
+        hipLaunchKernelGGL(guarded_fill64_kernel, dim3(fb), dim3(256), 0, s, (const Ctrl*)ctrl, acc64, numel);
+        hipLaunchKernelGGL(guarded_flat_kernel<true>, dim3(fb), dim3(256), 0, s, P, I, (const Ctrl*)ctrl, out, acc64);
+        hipLaunchKernelGGL(guarded_convert_kernel, dim3(fb), dim3(256), 0, s, (const Ctrl*)ctrl,
+                           (const unsigned long long*)acc64, out, numel);
+    } else {
+        hipLaunchKernelGGL(guarded_fill_kernel, dim3(fb), dim3(256), 0, s, (const Ctrl*)ctrl, out, numel);
+        hipLaunchKernelGGL(guarded_flat_kernel<false>, dim3(fb), dim3(256), 0, s, P, I, (const Ctrl*)ctrl, out,
+                           (unsigned long long*)nullptr);
+    }
+    if (hipGetLastError() != hipSuccess) return PDS_ERR_HIP;
     return PDS_OK;
 }
 

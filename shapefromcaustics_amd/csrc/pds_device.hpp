@@ -56,9 +56,76 @@ __device__ __forceinline__ bool window_ellipse_ok(int32_t xo, int32_t yo, int32_
     return a + b <= 1.0;
 }
 
-// Pixel-corner coordinate of pixel p along an axis of n pixels, in float with
-// the reference's operation order (2 * p / n - 1, IEEE division).
-__device__ __forceinline__ float pixel_coord(int32_t p, float n) { return 2.0f * (float)p / n - 1.0f; }
+// Photon-local pixel frame.  The reference evaluates the kernel at
+// dx = (2 px / W - 1) - x in float (photon_differentials.cu:98-100), which for
+// a non-power-of-two W rounds the pixel coordinate by up to 6e-8, a visible
+// error for footprints a fraction of a pixel wide.  Here the photon centre in
+// pixel units, W (x + 1) / 2, is split in double into an integer ix and a
+// fraction fx in [-1/2, 1/2], so that
+//     dx * W / 2 = (px - ix) - fx            (one float rounding),
+// and the matrix is pre-scaled once per photon, M' = M diag(2/W, 2/H), so that
+//     (u, v) = M' (px - ix - fx, py - iy - fy)  equals  M (dx, dy).
+// The backward rescales its sums by W/2, H/2, 2/W, 2/H once per photon.
+struct Frame {
+    int32_t ix, iy;
+    float fx, fy;
+    float a00, a01, a10, a11;   // M'
+};
+
+__device__ __forceinline__ Frame photon_frame(float cx, float cy, float m00, float m01, float m10, float m11,
+                                              float poison, int32_t W, int32_t H, float inv_hw, float inv_hh) {
+    Frame f;
+    const double hw = 0.5 * (double)W, hh = 0.5 * (double)H;
+    const double px = ((double)cx + 1.0) * hw, py = ((double)cy + 1.0) * hh;  // exact in double
+    // photons that reach this point have a non-empty window on the image, so a
+    // finite centre lies within a few windows of it and fits an int32
+    const double rx = (px == px && fabs(px) < 1.0e9) ? rint(px) : 0.0;
+    const double ry = (py == py && fabs(py) < 1.0e9) ? rint(py) : 0.0;
+    f.ix = (int32_t)rx;
+    f.iy = (int32_t)ry;
+    f.fx = (float)(px - rx) + poison;   // NaN/inf centre or NaN poison propagate
+    f.fy = (float)(py - ry);
+    f.a00 = m00 * inv_hw;               // inv_hw = 2 / W
+    f.a01 = m01 * inv_hh;
+    f.a10 = m10 * inv_hw;
+    f.a11 = m11 * inv_hh;
+    return f;
+}
+
+// Kahan's 2x2 determinant with FMA: a d - b c to about 1.5 ulp even under
+// cancellation.
+__device__ __forceinline__ float det2(float a, float b, float c, float d) {
+    const float w = b * c;
+    const float e = fmaf(-b, c, w);
+    const float f = fmaf(a, d, -w);
+    return f + e;
+}
+
+// Half extents, in pixels, of the ellipse |A d| < 1 (A = the frame matrix):
+// the row norms of A^-1, widened by 1e-4 relative plus 1e-4 px, far more than
+// the float error of the extents or of q near the boundary.  Returns false
+// for a singular or non-finite matrix (the caller then keeps the window).
+__device__ __forceinline__ bool ellipse_half_extents(const Frame& f, float& hx, float& hy) {
+    const float det = fabsf(det2(f.a00, f.a01, f.a10, f.a11));
+    if (!(det > 0.0f) || !(det < INFINITY)) return false;
+    const float inv = __builtin_amdgcn_rcpf(det);
+    hx = __builtin_amdgcn_sqrtf(fmaf(f.a11, f.a11, f.a01 * f.a01)) * inv * 1.0001f + 1e-4f;
+    hy = __builtin_amdgcn_sqrtf(fmaf(f.a10, f.a10, f.a00 * f.a00)) * inv * 1.0001f + 1e-4f;
+    if (!(hx < 1.0e6f)) hx = 1.0e6f;
+    if (!(hy < 1.0e6f)) hy = 1.0e6f;
+    return true;
+}
+
+// Clip [x0, x1] x [y0, y1] (absolute pixels) to the ellipse's bounding box.
+__device__ __forceinline__ void clip_to_ellipse(const Frame& f, int32_t& x0, int32_t& x1, int32_t& y0,
+                                                int32_t& y1) {
+    float hx, hy;
+    if (!ellipse_half_extents(f, hx, hy)) return;
+    x0 = max(x0, f.ix + (int32_t)ceilf(f.fx - hx));
+    x1 = min(x1, f.ix + (int32_t)floorf(f.fx + hx));
+    y0 = max(y0, f.iy + (int32_t)ceilf(f.fy - hy));
+    y1 = min(y1, f.iy + (int32_t)floorf(f.fy + hy));
+}
 
 __device__ __forceinline__ float silverman(float q) {
     const float t = 1.0f - q;

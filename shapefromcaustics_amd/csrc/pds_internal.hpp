@@ -27,6 +27,7 @@ struct PhotonArgs {
 struct ImageArgs {
     int32_t C, H, W, cap;
     float fw, fh;          // (float)W, (float)H as in photon_differentials.cu:76-77
+    float inv_hw, inv_hh;  // 2 / W, 2 / H
     int32_t ell_trivial;   // reference ellipse test (:93) is a no-op for every window offset
 };
 
@@ -36,17 +37,38 @@ struct GradArgs {
     float* gM;   // [2, 3, N]
 };
 
-// Store the seven accumulated sums of photon i; gM[:,2] is always zero
-// (photon_differentials.cu:210-218).
-__device__ __forceinline__ void store_grads(const GradArgs& G, int64_t N, int64_t i, const float (&a)[7]) {
+// Accumulate the seven gradient terms of one pixel in the photon-local frame
+// (photon_differentials.cu:195-204 with dk * (u, v) / r = sc * (u, v)):
+//   a0 += K g;  a1 -= sc (a00 u + a10 v);  a2 -= sc (a01 u + a11 v);
+//   a3 += sc u dx;  a4 += sc u dy;  a5 += sc v dx;  a6 += sc v dy
+// where (dx, dy) are pixel offsets and a.. the pre-scaled matrix; store_grads
+// converts back to the reference's normalised units.
+__device__ __forceinline__ void accumulate_terms(float (&a)[7], float q, float g, float sc, float u, float v, float dx,
+                                                 float dy, const Frame& f) {
+    a[0] += silverman(q) * g;
+    a[1] -= sc * (f.a00 * u + f.a10 * v);
+    a[2] -= sc * (f.a01 * u + f.a11 * v);
+    a[3] += sc * u * dx;
+    a[4] += sc * u * dy;
+    a[5] += sc * v * dx;
+    a[6] += sc * v * dy;
+}
+
+// Store the seven sums of photon i in the reference's units and layout;
+// gM[:,2] is always zero (photon_differentials.cu:210-218).  Pixel offsets
+// are (W/2) dx and (H/2) dy, and a.. = M diag(2/W, 2/H), hence the factors.
+__device__ __forceinline__ void store_grads(const GradArgs& G, const ImageArgs& I, int64_t N, int64_t i,
+                                            const float (&a)[7]) {
+    const float hw = 0.5f * I.fw, hh = 0.5f * I.fh;
+    const float iw = 2.0f / I.fw, ih = 2.0f / I.fh;
     G.gE[i] = a[0];
-    G.gx[i] = a[1];
-    G.gx[N + i] = a[2];
-    G.gM[i] = a[3];
-    G.gM[N + i] = a[4];
+    G.gx[i] = a[1] * hw;
+    G.gx[N + i] = a[2] * hh;
+    G.gM[i] = a[3] * iw;
+    G.gM[N + i] = a[4] * ih;
     G.gM[2 * N + i] = 0.0f;
-    G.gM[3 * N + i] = a[5];
-    G.gM[4 * N + i] = a[6];
+    G.gM[3 * N + i] = a[5] * iw;
+    G.gM[4 * N + i] = a[6] * ih;
     G.gM[5 * N + i] = 0.0f;
 }
 
@@ -76,40 +98,41 @@ __device__ __forceinline__ int64_t wave_inclusive_sum(int64_t v) {
 
 // One photon's footprint box, prepared for the flattened walk.
 struct FlatRec {
-    float cx, cy, e, inv_cpr;
-    float m00, m01, m10, m11;
-    int32_t x0, y0, cpr, bw;   // box origin (image pixels), chunks per row, box width
-    int32_t plane;             // c * H * W
+    Frame f;
+    float e, inv_cpr;
+    int32_t kx0, ky0;          // box origin relative to (ix, iy)
+    int32_t cpr, bw;           // chunks per row, box width
+    int32_t base;              // flat index of the box origin pixel in [C,H,W]
     int32_t nitems;            // rows * chunks per row (0: nothing to do)
     int64_t start;             // exclusive prefix of nitems over the wavefront
 };
 
 // Per-wavefront LDS copy of the 64 records of the current batch.
 struct WaveRecs {
-    float4 A[64];      // cx, cy, e, inv_cpr
-    float4 B[64];      // m00, m01, m10, m11
-    int4 Cq[64];       // x0, y0, cpr, bw
-    int32_t plane[64];
+    float4 A[64];      // fx, fy, e, inv_cpr
+    float4 B[64];      // a00, a01, a10, a11
+    int4 Cq[64];       // kx0, ky0, cpr, bw
+    int32_t base[64];
     int64_t start[64];
     int32_t mark[64];  // record whose first item lands on this lane (-1: none)
 };
 
 // Footprint box of photon i: the reference window (photon_differentials.cu:80-96)
 // clipped to the image, then -- for photons whose inputs are all finite -- to
-// the bounding box of the ellipse |M d| < 1 widened by one pixel.  Pixels
-// outside that ellipse have K = 0 and contribute exactly nothing, so the
-// result is unchanged.  Non-finite photons keep the whole window so that NaN
+// the bounding box of the ellipse |M d| < 1 (with a 1e-4 relative safety
+// margin).  Pixels outside that ellipse have K = 0 and contribute exactly
+// nothing, so the result is unchanged.  Non-finite photons keep the whole window so that NaN
 // propagates into their gradients the way it does in the reference.
 template <int CH>
 __device__ __forceinline__ void setup_flat_rec(const PhotonArgs& P, const ImageArgs& I, int64_t i, bool forward,
                                                FlatRec& r) {
     r.nitems = 0;
-    r.x0 = r.y0 = r.bw = 0;
+    r.kx0 = r.ky0 = r.bw = r.base = 0;
     r.cpr = 1;
     r.inv_cpr = 1.0f;
-    r.plane = 0;
-    r.cx = r.cy = r.e = 0.f;
-    r.m00 = r.m01 = r.m10 = r.m11 = 0.f;
+    r.e = 0.f;
+    r.f.ix = r.f.iy = 0;
+    r.f.fx = r.f.fy = r.f.a00 = r.f.a01 = r.f.a10 = r.f.a11 = 0.f;
     if (i >= P.N) return;
     const int64_t c = P.cp[i];
     if (c < 0 || c >= I.C) return;
@@ -125,45 +148,29 @@ __device__ __forceinline__ void setup_flat_rec(const PhotonArgs& P, const ImageA
     int64_t y1 = min((int64_t)win.pyc + win.ry, (int64_t)I.H - 1);
     if (x0 > x1 || y0 > y1) return;
     const float m00 = P.M[i], m01 = P.M[N + i], m10 = P.M[3 * N + i], m11 = P.M[4 * N + i];
-    const float z = P.M[2 * N + i] * 0.0f + P.M[5 * N + i] * 0.0f;  // NaN iff M[:,2] is not finite
+    const float poison = P.M[2 * N + i] * 0.0f + P.M[5 * N + i] * 0.0f;  // NaN iff M[:,2] is not finite
+    const Frame f = photon_frame(cx, cy, m00, m01, m10, m11, poison, I.W, I.H, I.inv_hw, I.inv_hh);
     const bool finite = isfinite(cx) && isfinite(cy) && isfinite(e) && isfinite(m00) && isfinite(m01) &&
-                        isfinite(m10) && isfinite(m11) && z == 0.0f;
+                        isfinite(m10) && isfinite(m11) && poison == 0.0f;
     if (finite) {
-        const double a = m00, b = m01, cc = m10, d = m11;
-        const double det = a * d - b * cc;
-        if (det != 0.0) {
-            const double ad = fabs(det);
-            const double hx = sqrt(d * d + b * b) / ad * (0.5 * (double)I.W);
-            const double hy = sqrt(cc * cc + a * a) / ad * (0.5 * (double)I.H);
-            const double cxp = ((double)cx + 1.0) * 0.5 * (double)I.W;
-            const double cyp = ((double)cy + 1.0) * 0.5 * (double)I.H;
-            const double lx = fmin(fmax(floor(cxp - hx) - 1.0, (double)x0), (double)x1 + 1.0);
-            const double hxp = fmax(fmin(ceil(cxp + hx) + 1.0, (double)x1), (double)x0 - 1.0);
-            const double ly = fmin(fmax(floor(cyp - hy) - 1.0, (double)y0), (double)y1 + 1.0);
-            const double hyp = fmax(fmin(ceil(cyp + hy) + 1.0, (double)y1), (double)y0 - 1.0);
-            x0 = (int64_t)lx;
-            x1 = (int64_t)hxp;
-            y0 = (int64_t)ly;
-            y1 = (int64_t)hyp;
-            if (x0 > x1 || y0 > y1) return;
-        }
+        int32_t a0 = (int32_t)x0, a1 = (int32_t)x1, b0 = (int32_t)y0, b1 = (int32_t)y1;
+        clip_to_ellipse(f, a0, a1, b0, b1);
+        if (a0 > a1 || b0 > b1) return;
+        x0 = a0;
+        x1 = a1;
+        y0 = b0;
+        y1 = b1;
     }
     const int32_t bw = (int32_t)(x1 - x0 + 1), bh = (int32_t)(y1 - y0 + 1);
+    r.f = f;
+    r.e = e;
     r.cpr = (bw + CH - 1) / CH;
     r.inv_cpr = 1.0f / (float)r.cpr;
     r.nitems = bh * r.cpr;
-    r.x0 = (int32_t)x0;
-    r.y0 = (int32_t)y0;
+    r.kx0 = (int32_t)x0 - f.ix;
+    r.ky0 = (int32_t)y0 - f.iy;
     r.bw = bw;
-    r.plane = (int32_t)(c * I.H * I.W);
-    // a NaN from M[:,2] enters through the centre so that u and v are NaN
-    r.cx = finite ? cx : cx + z;
-    r.cy = cy;
-    r.e = e;
-    r.m00 = m00;
-    r.m01 = m01;
-    r.m10 = m10;
-    r.m11 = m11;
+    r.base = (int32_t)((c * I.H + y0) * I.W + x0);
 }
 
 // Publish this lane's record to the wavefront's LDS table; returns the total
@@ -172,10 +179,10 @@ __device__ __forceinline__ int64_t publish_recs(WaveRecs& R, FlatRec& r, int lan
     const int64_t incl = wave_inclusive_sum((int64_t)r.nitems);
     r.start = incl - r.nitems;
     const int64_t total = __shfl(incl, 63);
-    R.A[lane] = make_float4(r.cx, r.cy, r.e, r.inv_cpr);
-    R.B[lane] = make_float4(r.m00, r.m01, r.m10, r.m11);
-    R.Cq[lane] = make_int4(r.x0, r.y0, r.cpr, r.bw);
-    R.plane[lane] = r.plane;
+    R.A[lane] = make_float4(r.f.fx, r.f.fy, r.e, r.inv_cpr);
+    R.B[lane] = make_float4(r.f.a00, r.f.a01, r.f.a10, r.f.a11);
+    R.Cq[lane] = make_int4(r.kx0, r.ky0, r.cpr, r.bw);
+    R.base[lane] = r.base;
     R.start[lane] = r.start;
     wave_lds_fence();
     return total;
@@ -184,11 +191,12 @@ __device__ __forceinline__ int64_t publish_recs(WaveRecs& R, FlatRec& r, int lan
 // One work item: up to CH consecutive pixels of one row of one photon's box.
 struct FlatItem {
     int32_t rec;             // lane (record) the item belongs to
-    int32_t px, py;          // first pixel
+    int32_t kx;              // x offset of the first pixel from the photon's ix
     int32_t n;               // pixels in this item (1..CH)
-    int64_t pix;             // flat index of the first pixel in the [C,H,W] image
-    float cx, dy, e;
-    float m00, m01, m10, m11;
+    int32_t pix;             // flat index of the first pixel in the [C,H,W] image
+    float dy;                // row offset in pixels, (py - iy) - fy
+    float e;
+    Frame f;                 // fx, a.. valid (ix, iy, fy unused)
 };
 
 // Walk all work items of the batch, 64 per step.  The lane -> record map of a
@@ -219,21 +227,55 @@ __device__ __forceinline__ void walk_items(WaveRecs& R, const FlatRec& r, const 
             int32_t row = (int32_t)(((float)local + 0.5f) * A.w);
             if (local >= (1 << 20)) row = local / Q.z;  // the float quotient is exact below 2^20
             const int32_t chunk = local - row * Q.z;
-            it.px = Q.x + chunk * CH;
-            it.py = Q.y + row;
+            it.kx = Q.x + chunk * CH;
             it.n = min(CH, Q.w - chunk * CH);
-            it.pix = (int64_t)R.plane[m] + (int64_t)it.py * I.W + it.px;
-            it.cx = A.x;
-            it.dy = pixel_coord(it.py, I.fh) - A.y;
+            it.pix = R.base[m] + row * I.W + chunk * CH;
+            it.dy = (float)(Q.y + row) - A.y;
             it.e = A.z;
-            it.m00 = B.x;
-            it.m01 = B.y;
-            it.m10 = B.z;
-            it.m11 = B.w;
+            it.f.fx = A.x;
+            it.f.a00 = B.x;
+            it.f.a01 = B.y;
+            it.f.a10 = B.z;
+            it.f.a11 = B.w;
             body(it);
         }
         wave_lds_fence();
     }
+}
+
+// Per-photon setup of the plain (reference-schedule) kernels: the reference
+// window (photon_differentials.cu:74-89) clipped to the image, and the frame.
+struct PlainSetup {
+    bool ok;
+    int32_t c;
+    float e;
+    Frame f;
+    Window win;
+    int64_t x0, x1, y0, y1;  // window clipped to the image (inclusive)
+};
+
+__device__ __forceinline__ PlainSetup plain_setup(const PhotonArgs& P, const ImageArgs& I, int64_t i) {
+    PlainSetup s;
+    s.ok = false;
+    const int64_t c = P.cp[i];
+    if (c < 0 || c >= I.C) return s;  // reference skips c >= C (:75); negative channels skipped too
+    s.c = (int32_t)c;
+    const float cx = P.xp[i], cy = P.xp[P.N + i];
+    s.win = photon_window(P.radius[i], cx, cy, I.W, I.H, I.cap);
+    if (s.win.rx < 0 || s.win.ry < 0) return s;
+    s.x0 = max((int64_t)s.win.pxc - s.win.rx, (int64_t)0);
+    s.x1 = min((int64_t)s.win.pxc + s.win.rx, (int64_t)I.W - 1);
+    s.y0 = max((int64_t)s.win.pyc - s.win.ry, (int64_t)0);
+    s.y1 = min((int64_t)s.win.pyc + s.win.ry, (int64_t)I.H - 1);
+    s.ok = s.x0 <= s.x1 && s.y0 <= s.y1;
+    if (!s.ok) return s;
+    const int64_t N = P.N;
+    // the reference multiplies M[:,2] by the zero third coordinate (:103): NaN if not finite
+    const float poison = P.M[2 * N + i] * 0.0f + P.M[5 * N + i] * 0.0f;
+    s.f = photon_frame(cx, cy, P.M[i], P.M[N + i], P.M[3 * N + i], P.M[4 * N + i], poison, I.W, I.H, I.inv_hw,
+                       I.inv_hh);
+    s.e = P.E[i];
+    return s;
 }
 
 // launchers (pds_kernels.hip)

@@ -55,54 +55,21 @@ hipError_t launch_fill_zero(float* p, int64_t n, hipStream_t s) {
 // ---------------------------------------------------------------------------
 // plain kernels: the reference's schedule
 // ---------------------------------------------------------------------------
-struct PlainSetup {
-    bool ok;
-    int32_t c;
-    float cx, cy, e, m00, m01, m10, m11, z0, z1;
-    Window win;
-    int64_t x0, x1, y0, y1;  // window clipped to the image (inclusive)
-};
-
-__device__ __forceinline__ PlainSetup plain_setup(const PhotonArgs& P, const ImageArgs& I, int64_t i) {
-    PlainSetup s;
-    s.ok = false;
-    const int64_t c = P.cp[i];
-    if (c < 0 || c >= I.C) return s;  // reference skips c >= C (:75); negative channels skipped too
-    s.c = (int32_t)c;
-    s.cx = P.xp[i];
-    s.cy = P.xp[P.N + i];
-    s.win = photon_window(P.radius[i], s.cx, s.cy, I.W, I.H, I.cap);
-    if (s.win.rx < 0 || s.win.ry < 0) return s;
-    const int64_t N = P.N;
-    s.m00 = P.M[i];
-    s.m01 = P.M[N + i];
-    s.m10 = P.M[3 * N + i];
-    s.m11 = P.M[4 * N + i];
-    s.z0 = P.M[2 * N + i] * 0.0f;  // reference multiplies M[:,2] by the zero third coordinate (:103)
-    s.z1 = P.M[5 * N + i] * 0.0f;
-    s.e = P.E[i];
-    s.x0 = max((int64_t)s.win.pxc - s.win.rx, (int64_t)0);
-    s.x1 = min((int64_t)s.win.pxc + s.win.rx, (int64_t)I.W - 1);
-    s.y0 = max((int64_t)s.win.pyc - s.win.ry, (int64_t)0);
-    s.y1 = min((int64_t)s.win.pyc + s.win.ry, (int64_t)I.H - 1);
-    s.ok = s.x0 <= s.x1 && s.y0 <= s.y1;
-    return s;
-}
-
 __global__ __launch_bounds__(256) void fwd_plain_kernel(PhotonArgs P, ImageArgs I, float* __restrict__ out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.N) return;
     const PlainSetup s = plain_setup(P, I, i);
     if (!s.ok) return;
     float* plane = out + (int64_t)s.c * I.H * I.W;
+    const Frame& f = s.f;
     for (int64_t py = s.y0; py <= s.y1; ++py) {
-        const float dy = pixel_coord((int32_t)py, I.fh) - s.cy;
+        const float dy = (float)((int32_t)py - f.iy) - f.fy;
         for (int64_t px = s.x0; px <= s.x1; ++px) {
             if (!I.ell_trivial && !window_ellipse_ok((int32_t)(px - s.win.pxc), (int32_t)(py - s.win.pyc), I.W, I.H))
                 continue;
-            const float dx = pixel_coord((int32_t)px, I.fw) - s.cx;
-            const float u = s.m00 * dx + s.m01 * dy + s.z0;
-            const float v = s.m10 * dx + s.m11 * dy + s.z1;
+            const float dx = (float)((int32_t)px - f.ix) - f.fx;
+            const float u = f.a00 * dx + f.a01 * dy;
+            const float v = f.a10 * dx + f.a11 * dy;
             const float val = silverman(u * u + v * v) * s.e;
             if (val > 0.0f) unsafeAtomicAdd(&plane[py * I.W + px], val);
         }
@@ -117,29 +84,24 @@ __global__ __launch_bounds__(256) void bwd_plain_kernel(const float* __restrict_
     float a[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (s.ok) {
         const float* plane = grad + (int64_t)s.c * I.H * I.W;
+        const Frame& f = s.f;
         for (int64_t py = s.y0; py <= s.y1; ++py) {
-            const float dy = pixel_coord((int32_t)py, I.fh) - s.cy;
+            const float dy = (float)((int32_t)py - f.iy) - f.fy;
             for (int64_t px = s.x0; px <= s.x1; ++px) {
                 if (!I.ell_trivial &&
                     !window_ellipse_ok((int32_t)(px - s.win.pxc), (int32_t)(py - s.win.pyc), I.W, I.H))
                     continue;
-                const float dx = pixel_coord((int32_t)px, I.fw) - s.cx;
-                const float u = s.m00 * dx + s.m01 * dy + s.z0;
-                const float v = s.m10 * dx + s.m11 * dy + s.z1;
+                const float dx = (float)((int32_t)px - f.ix) - f.fx;
+                const float u = f.a00 * dx + f.a01 * dy;
+                const float v = f.a10 * dx + f.a11 * dy;
                 const float q = u * u + v * v;
                 const float g = plane[py * I.W + px];
                 const float sc = dk_over_r(q, s.e * g);
-                a[0] += silverman(q) * g;
-                a[1] -= sc * (s.m00 * u + s.m10 * v);
-                a[2] -= sc * (s.m01 * u + s.m11 * v);
-                a[3] += sc * u * dx;
-                a[4] += sc * u * dy;
-                a[5] += sc * v * dx;
-                a[6] += sc * v * dy;
+                accumulate_terms(a, q, g, sc, u, v, dx, dy, f);
             }
         }
     }
-    store_grads(G, P.N, i, a);
+    store_grads(G, I, P.N, i, a);
 }
 
 // ---------------------------------------------------------------------------
@@ -163,9 +125,9 @@ __global__ __launch_bounds__(256) void fwd_flat_kernel(PhotonArgs P, ImageArgs I
 #pragma unroll
             for (int k = 0; k < CH; ++k) {
                 if (k < it.n) {
-                    const float dx = pixel_coord(it.px + k, I.fw) - it.cx;
-                    const float u = it.m00 * dx + it.m01 * it.dy;
-                    const float v = it.m10 * dx + it.m11 * it.dy;
+                    const float dx = (float)(it.kx + k) - it.f.fx;
+                    const float u = it.f.a00 * dx + it.f.a01 * it.dy;
+                    const float v = it.f.a10 * dx + it.f.a11 * it.dy;
                     const float val = silverman(u * u + v * v) * it.e;
                     if (val > 0.0f) unsafeAtomicAdd(&out[it.pix + k], val);
                 }
@@ -193,22 +155,19 @@ __global__ __launch_bounds__(256) void bwd_flat_kernel(const float* __restrict__
         int64_t total = publish_recs(R, r, lane);
         walk_items<CH>(R, r, I, lane, total, [&](const FlatItem& it) {
             float a[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            // issue every gather of the item before using any of them
+            float g[CH];
+#pragma unroll
+            for (int k = 0; k < CH; ++k) g[k] = grad[it.pix + (k < it.n ? k : 0)];
 #pragma unroll
             for (int k = 0; k < CH; ++k) {
                 if (k < it.n) {
-                    const float dx = pixel_coord(it.px + k, I.fw) - it.cx;
-                    const float u = it.m00 * dx + it.m01 * it.dy;
-                    const float v = it.m10 * dx + it.m11 * it.dy;
+                    const float dx = (float)(it.kx + k) - it.f.fx;
+                    const float u = it.f.a00 * dx + it.f.a01 * it.dy;
+                    const float v = it.f.a10 * dx + it.f.a11 * it.dy;
                     const float q = u * u + v * v;
-                    const float g = grad[it.pix + k];
-                    const float sc = dk_over_r(q, it.e * g);
-                    a[0] += silverman(q) * g;
-                    a[1] -= sc * (it.m00 * u + it.m10 * v);
-                    a[2] -= sc * (it.m01 * u + it.m11 * v);
-                    a[3] += sc * u * dx;
-                    a[4] += sc * u * it.dy;
-                    a[5] += sc * v * dx;
-                    a[6] += sc * v * it.dy;
+                    const float sc = dk_over_r(q, it.e * g[k]);
+                    accumulate_terms(a, q, g[k], sc, u, v, dx, it.dy, it.f);
                 }
             }
 #pragma unroll
@@ -221,7 +180,7 @@ __global__ __launch_bounds__(256) void bwd_flat_kernel(const float* __restrict__
             s[k] = acc[w][k][lane];
             acc[w][k][lane] = 0.f;
         }
-        if (i < P.N) store_grads(G, P.N, i, s);
+        if (i < P.N) store_grads(G, I, P.N, i, s);
         wave_lds_fence();
     }
 }

@@ -23,7 +23,7 @@ def _declared_functions():
 def test_library_exports_every_declared_symbol():
     lib = _native.lib()
     names = _declared_functions()
-    assert {"pds_forward_f32", "pds_backward_f32", "pds_workspace_bytes", "pds_last_error"} <= set(names)
+    assert {"pds_forward_f32", "pds_backward_f32", "pds_forward_workspace_bytes", "pds_last_error"} <= set(names)
     for name in names:
         assert hasattr(lib, name), name
     assert set(names) <= set(_native.SIGNATURES)
@@ -53,7 +53,7 @@ def test_forward_argument_errors(kw, code, msg):
 
 
 def test_forward_workspace_too_small():
-    need = _native.lib().pds_workspace_bytes(1 << 16, 3, 256, 256, 8, _native.MODES["binned"], None)
+    need = _native.lib().pds_forward_workspace_bytes(1 << 16, 3, 256, 256, 8, _native.MODES["binned"], None)
     if need == 0:
         pytest.skip("binned path needs no workspace")
     assert _fwd(N=1 << 16, C=3, H=256, W=256, cap=8, mode=2, ws=None, wsb=0) == _native.PDS_ERR_WORKSPACE
