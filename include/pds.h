@@ -93,6 +93,31 @@ int pds_backward_f32(const float* grad, const float* Ep, const float* xp, const 
                      float* grad_Ep, float* grad_xp, float* grad_Mp, void* workspace,
                      size_t workspace_bytes, void* stream);
 
+/* ---- forward/backward pair sharing one binning -------------------------
+ * The autograd path runs the forward and then the backward of the same
+ * photons.  pds_forward_bins_f32 leaves the photons' tile binning in the
+ * caller-owned ``bins`` buffer and pds_backward_bins_f32 reuses it instead of
+ * binning again.  The inputs must be unchanged between the two calls.  No
+ * counterpart in the reference (its backward re-walks every window). */
+
+/* Bytes of ``bins`` for this problem; 0 if it cannot be binned (tiny image,
+ * max_pixel_radius > 63, too many tiles, plain mode): use the plain pair. */
+size_t pds_bins_bytes(int64_t N, int32_t C, int32_t H, int32_t W, int32_t max_pixel_radius, int32_t mode,
+                      const pds_tuning* t);
+
+/* pds_forward_f32 that keeps the binning in ``bins``. */
+int pds_forward_bins_f32(const float* Ep, const float* xp, const float* Mp, const int64_t* cp,
+                         const float* radius, int64_t N, int32_t C, int32_t H, int32_t W,
+                         int32_t max_pixel_radius, int32_t mode, const pds_tuning* t, float* out,
+                         void* bins, size_t bins_bytes, void* stream);
+
+/* pds_backward_f32 reusing ``bins`` from pds_forward_bins_f32 on the same inputs. */
+int pds_backward_bins_f32(const float* grad, const float* Ep, const float* xp, const float* Mp,
+                          const int64_t* cp, const float* radius, int64_t N, int32_t C, int32_t H,
+                          int32_t W, int32_t max_pixel_radius, int32_t mode, const pds_tuning* t,
+                          float* grad_Ep, float* grad_xp, float* grad_Mp, void* bins, size_t bins_bytes,
+                          void* stream);
+
 /* Message of the calling thread's last failed call ("" if none). */
 const char* pds_last_error(void);
 

@@ -81,17 +81,20 @@ int pds_oracle_forward_f64(const float* E, const float* xp, const float* M, cons
 
 /* Backward, FP64, over the photon subset idx[0..n_idx) (idx == NULL: all N).
  * Outputs are indexed by position in the subset: gE[k], gx[2][n], gM[6][n]
- * with n = n_idx.  a* (may be NULL) receive sums of |term| per component. */
+ * with n = n_idx.  a* (may be NULL) receive sums of |term| per component and
+ * c* (may be NULL) a first-order bound of the FP32 evaluation error of each
+ * component (conditioning of the photon's gradient). */
 int pds_oracle_backward_f64(const float* grad, const float* E, const float* xp, const float* M,
                             const int64_t* cp, const float* radius, int64_t N, int32_t C,
                             int32_t H, int32_t W, int32_t cap, const int64_t* idx, int64_t n_idx,
-                            double* gE, double* gx, double* gM, double* aE, double* ax, double* aM)
+                            double* gE, double* gx, double* gM, double* aE, double* ax, double* aM,
+                            double* cE, double* cx_, double* cM)
 {
     if (N < 0 || C <= 0 || H <= 0 || W <= 0) return -1;
     int64_t n = idx ? n_idx : N;
     for (int64_t k = 0; k < n; ++k) {
         int64_t i = idx ? idx[k] : k;
-        double acc[9] = {0}, mag[9] = {0};
+        double acc[9] = {0}, mag[9] = {0}, cond[9] = {0};
         int64_t c = cp[i];
         if (c >= 0 && c < C) {
             int32_t rx, ry, pxc, pyc;
@@ -136,6 +139,34 @@ int pds_oracle_backward_f64(const float* grad, const float* E, const float* xp, 
                             acc[j] += t[j];
                             mag[j] += fabs(t[j]);
                         }
+                        /* First-order bound of what FP32 evaluation of this pixel's
+                         * terms can be off by: rounding of u, v (a few ulps of
+                         * their products, plus the photon centre held to 2^-25 px)
+                         * and of q, propagated through each term's derivative.
+                         * Large only where a footprint barely contains a pixel
+                         * (1 - q tiny), where the result is ill-conditioned. */
+                        if (q < 1.0 && r > 0.0) {
+                            const double ur = 8.0 * 5.9604644775390625e-08; /* 8 * 2^-24 */
+                            const double ddx = 2.9802322387695312e-08 * 2.0 / (double)W;
+                            const double ddy = 2.9802322387695312e-08 * 2.0 / (double)H;
+                            double eu = ur * (fabs(m00 * dx) + fabs(m01 * dy)) + fabs(m00) * ddx + fabs(m01) * ddy;
+                            double ev = ur * (fabs(m10 * dx) + fabs(m11 * dy)) + fabs(m10) * ddx + fabs(m11) * ddy;
+                            double eq = 2.0 * fabs(u) * eu + 2.0 * fabs(v) * ev + ur * q;
+                            double sv = -12.0 / PDS_PI * (1.0 - q) * e * g; /* dk / r */
+                            double sq = 12.0 / PDS_PI * e * g;               /* d(sv)/dq */
+                            double gu = m00 * u + m10 * v, gv = m01 * u + m11 * v;
+                            double d[9];
+                            d[0] = 6.0 / PDS_PI * (1.0 - q) * fabs(g) * eq;
+                            d[1] = fabs(sq * gu) * eq + fabs(sv) * (fabs(m00) * eu + fabs(m10) * ev);
+                            d[2] = fabs(sq * gv) * eq + fabs(sv) * (fabs(m01) * eu + fabs(m11) * ev);
+                            d[3] = fabs(sq * u * dx) * eq + fabs(sv * dx) * eu + fabs(sv * u) * ddx;
+                            d[4] = fabs(sq * u * dy) * eq + fabs(sv * dy) * eu + fabs(sv * u) * ddy;
+                            d[5] = 0.0;
+                            d[6] = fabs(sq * v * dx) * eq + fabs(sv * dx) * ev + fabs(sv * v) * ddx;
+                            d[7] = fabs(sq * v * dy) * eq + fabs(sv * dy) * ev + fabs(sv * v) * ddy;
+                            d[8] = 0.0;
+                            for (int j = 0; j < 9; ++j) cond[j] += d[j];
+                        }
                     }
                 }
             }
@@ -151,6 +182,13 @@ int pds_oracle_backward_f64(const float* grad, const float* E, const float* xp, 
         }
         if (aM)
             for (int j = 0; j < 6; ++j) aM[(size_t)j * n + k] = mag[3 + j];
+        if (cE) cE[k] = cond[0];
+        if (cx_) {
+            cx_[k] = cond[1];
+            cx_[n + k] = cond[2];
+        }
+        if (cM)
+            for (int j = 0; j < 6; ++j) cM[(size_t)j * n + k] = cond[3 + j];
     }
     return 0;
 }

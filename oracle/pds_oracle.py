@@ -25,6 +25,7 @@ Array layout is the reference's (``model/photon_differential.py:9``,
 """
 from __future__ import annotations
 
+import collections
 import ctypes
 import os
 import subprocess
@@ -60,7 +61,8 @@ def _lib(name: str):
             lib.pds_oracle_backward_f64.argtypes = [_f32p, _f32p, _f32p, _f32p, _i64p, _f32p, ctypes.c_int64,
                                                     ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                                     ctypes.c_int32, _i64p, ctypes.c_int64,
-                                                    _f64p, _f64p, _f64p, _f64p, _f64p, _f64p]
+                                                    _f64p, _f64p, _f64p, _f64p, _f64p, _f64p,
+                                                    _f64p, _f64p, _f64p]
             lib.pds_oracle_photon_loss_f64.argtypes = [_f32p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                                        ctypes.c_int32, ctypes.c_int64, ctypes.c_double,
                                                        ctypes.c_double, ctypes.c_double, _f64p, ctypes.c_double]
@@ -104,13 +106,18 @@ def forward_f64(E, xp, M, cp, radius, size, cap) -> np.ndarray:
     return out
 
 
-def backward_f64(grad, E, xp, M, cp, radius, cap, idx=None):
+OracleGrads = collections.namedtuple("OracleGrads", "gE gx gM aE ax aM cE cx cM")
+
+
+def backward_f64(grad, E, xp, M, cp, radius, cap, idx=None) -> OracleGrads:
     """FP64 oracle backward.
 
-    Returns ``(gE, gx, gM, aE, ax, aM)`` over the photon subset ``idx`` (all
-    photons if None), shaped ``[n]``, ``[2, n]``, ``[2, 3, n]``; the ``a*``
-    arrays hold the sum of ``|term|`` of every component (gradient tolerance
-    scale).
+    Returns ``OracleGrads(gE, gx, gM, aE, ax, aM, cE, cx, cM)`` over the photon
+    subset ``idx`` (all photons if None), shaped ``[n]``, ``[2, n]``,
+    ``[2, 3, n]``.  ``a*`` hold the sum of ``|term|`` of every component (the
+    gradient tolerance scale), ``c*`` a first-order bound of how far an FP32
+    evaluation of the same formulae can be off (large only for photons whose
+    footprint barely contains a pixel).
     """
     E, xp, M, cp, radius, n_all = _inputs(E, xp, M, cp, radius)
     grad = np.ascontiguousarray(grad, dtype=np.float32)
@@ -120,15 +127,30 @@ def backward_f64(grad, E, xp, M, cp, radius, cap, idx=None):
     else:
         idx = np.ascontiguousarray(idx, dtype=np.int64).reshape(-1)
         n, idx_p = idx.shape[0], _p(idx, _i64p)
-    gE, aE = np.empty(n), np.empty(n)
-    gx, ax = np.empty((2, n)), np.empty((2, n))
-    gM, aM = np.empty((2, 3, n)), np.empty((2, 3, n))
+    gE, aE, cE = np.empty(n), np.empty(n), np.empty(n)
+    gx, ax, cx = np.empty((2, n)), np.empty((2, n)), np.empty((2, n))
+    gM, aM, cM = np.empty((2, 3, n)), np.empty((2, 3, n)), np.empty((2, 3, n))
     rc = _lib("libpds_oracle.so").pds_oracle_backward_f64(
         _p(grad, _f32p), _p(E, _f32p), _p(xp, _f32p), _p(M, _f32p), _p(cp, _i64p), _p(radius, _f32p), n_all,
         C, H, W, int(cap), idx_p, n, _p(gE, _f64p), _p(gx, _f64p), _p(gM, _f64p), _p(aE, _f64p),
-        _p(ax, _f64p), _p(aM, _f64p))
+        _p(ax, _f64p), _p(aM, _f64p), _p(cE, _f64p), _p(cx, _f64p), _p(cM, _f64p))
     assert rc == 0
-    return gE, gx, gM, aE, ax, aM
+    return OracleGrads(gE, gx, gM, aE, ax, aM, cE, cx, cM)
+
+
+def grads_within_tolerance(got, ref: OracleGrads, rtol=1e-3):
+    """Per component: |g - g*| <= rtol * sum|terms| + FP32 conditioning bound.
+
+    Returns ``(ok, worst)``: whether every entry passes, and the largest
+    ``|g - g*| / (rtol * sum|terms| + bound)`` (ok iff worst <= 1).
+    """
+    worst = 0.0
+    for g, r, a, c in zip(got, (ref.gE, ref.gx, ref.gM), (ref.aE, ref.ax, ref.aM), (ref.cE, ref.cx, ref.cM)):
+        g = np.asarray(g, dtype=np.float64)
+        err = np.abs(g - r)
+        bound = rtol * a + c + 1e-300
+        worst = max(worst, float(np.max(err / bound, initial=0.0)))
+    return worst <= 1.0, worst
 
 
 def photon_loss_f64(grad, c, E, x, M6, radius, cap) -> float:

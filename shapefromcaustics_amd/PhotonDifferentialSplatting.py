@@ -93,9 +93,11 @@ def _workspace(query, N, C, H, W, cap, mode, tun, device):
     return torch.empty(nbytes, dtype=torch.uint8, device=device), nbytes
 
 
-def pds_forward(Ep: torch.Tensor, xp: torch.Tensor, Mp: torch.Tensor, cp: torch.Tensor, radius: torch.Tensor,
-                output_size: tp.Sequence[int], max_pixel_radius: int, *, mode="auto", tuning=None):
-    """Photon-differential splat; returns ``[grid]`` with ``grid`` of shape ``output_size``."""
+def _tun_ref(tun):
+    return ctypes.byref(tun) if tun is not None else None
+
+
+def _forward(Ep, xp, Mp, cp, radius, output_size, max_pixel_radius, mode, tuning, keep_bins):
     _check_photons(Ep, xp, Mp, cp, radius)
     size = [int(s) for s in output_size]
     if len(size) != 3:
@@ -105,20 +107,24 @@ def pds_forward(Ep: torch.Tensor, xp: torch.Tensor, Mp: torch.Tensor, cp: torch.
     m = _mode(mode)
     tun = _native.tuning_struct(tuning)
     lib = _native.lib()
+    cap = int(max_pixel_radius)
     with torch.cuda.device(Ep.device):
         out = torch.empty(size, dtype=torch.float32, device=Ep.device)
-        ws, nbytes = _workspace(lib.pds_forward_workspace_bytes, N, C, H, W, int(max_pixel_radius), m, tun, Ep.device)
+        bins = None
+        if keep_bins and C * H * W > 0 and lib.pds_bins_bytes(N, C, H, W, cap, m, _tun_ref(tun)) > 0:
+            bins, nbytes = _workspace(lib.pds_bins_bytes, N, C, H, W, cap, m, tun, Ep.device)
+            fn = lib.pds_forward_bins_f32
+        else:
+            ws, nbytes = _workspace(lib.pds_forward_workspace_bytes, N, C, H, W, cap, m, tun, Ep.device)
+            fn = lib.pds_forward_f32
         stream = torch.cuda.current_stream(Ep.device).cuda_stream
-        rc = lib.pds_forward_f32(_ptr(Ep), _ptr(xp), _ptr(Mp), _ptr(cp), _ptr(radius), N, C, H, W,
-                                 int(max_pixel_radius), m, ctypes.byref(tun) if tun is not None else None,
-                                 _ptr(out), _ptr(ws), nbytes, ctypes.c_void_p(stream))
+        rc = fn(_ptr(Ep), _ptr(xp), _ptr(Mp), _ptr(cp), _ptr(radius), N, C, H, W, cap, m, _tun_ref(tun), _ptr(out),
+                _ptr(bins if bins is not None else ws), nbytes, ctypes.c_void_p(stream))
     _native.check(rc, "pds_forward")
-    return [out]
+    return out, bins
 
 
-def pds_backward(grad_pds: torch.Tensor, Ep: torch.Tensor, xp: torch.Tensor, Mp: torch.Tensor, cp: torch.Tensor,
-                 radius: torch.Tensor, max_pixel_radius: int, *, mode="auto", tuning=None):
-    """Adjoint of :func:`pds_forward`; returns ``[grad_Ep, grad_xp, grad_Mp]``."""
+def _backward(grad_pds, Ep, xp, Mp, cp, radius, max_pixel_radius, mode, tuning, bins=None):
     if not grad_pds.is_cuda:
         raise RuntimeError("grad_pds must be a CUDA tensor")
     if grad_pds.dim() != 3:
@@ -134,15 +140,30 @@ def pds_backward(grad_pds: torch.Tensor, Ep: torch.Tensor, xp: torch.Tensor, Mp:
     m = _mode(mode)
     tun = _native.tuning_struct(tuning)
     lib = _native.lib()
+    cap = int(max_pixel_radius)
     with torch.cuda.device(Ep.device):
         gE = torch.empty_like(Ep)
         gx = torch.empty_like(xp)
         gM = torch.empty_like(Mp)
-        ws, nbytes = _workspace(lib.pds_backward_workspace_bytes, N, C, H, W, int(max_pixel_radius), m, tun,
-                                Ep.device)
         stream = torch.cuda.current_stream(Ep.device).cuda_stream
-        rc = lib.pds_backward_f32(_ptr(grad_pds), _ptr(Ep), _ptr(xp), _ptr(Mp), _ptr(cp), _ptr(radius), N, C, H, W,
-                                  int(max_pixel_radius), m, ctypes.byref(tun) if tun is not None else None,
-                                  _ptr(gE), _ptr(gx), _ptr(gM), _ptr(ws), nbytes, ctypes.c_void_p(stream))
+        if bins is not None:
+            fn, ws, nbytes = lib.pds_backward_bins_f32, bins, bins.numel()
+        else:
+            fn = lib.pds_backward_f32
+            ws, nbytes = _workspace(lib.pds_backward_workspace_bytes, N, C, H, W, cap, m, tun, Ep.device)
+        rc = fn(_ptr(grad_pds), _ptr(Ep), _ptr(xp), _ptr(Mp), _ptr(cp), _ptr(radius), N, C, H, W, cap, m,
+                _tun_ref(tun), _ptr(gE), _ptr(gx), _ptr(gM), _ptr(ws), nbytes, ctypes.c_void_p(stream))
     _native.check(rc, "pds_backward")
     return [gE, gx, gM]
+
+
+def pds_forward(Ep: torch.Tensor, xp: torch.Tensor, Mp: torch.Tensor, cp: torch.Tensor, radius: torch.Tensor,
+                output_size: tp.Sequence[int], max_pixel_radius: int, *, mode="auto", tuning=None):
+    """Photon-differential splat; returns ``[grid]`` with ``grid`` of shape ``output_size``."""
+    return [_forward(Ep, xp, Mp, cp, radius, output_size, max_pixel_radius, mode, tuning, keep_bins=False)[0]]
+
+
+def pds_backward(grad_pds: torch.Tensor, Ep: torch.Tensor, xp: torch.Tensor, Mp: torch.Tensor, cp: torch.Tensor,
+                 radius: torch.Tensor, max_pixel_radius: int, *, mode="auto", tuning=None):
+    """Adjoint of :func:`pds_forward`; returns ``[grad_Ep, grad_xp, grad_Mp]``."""
+    return _backward(grad_pds, Ep, xp, Mp, cp, radius, max_pixel_radius, mode, tuning)

@@ -44,6 +44,11 @@ SIGNATURES = {
     "pds_forward_f32": ([_vp, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _i32, _i32, _i32, _tp, _vp, _vp, _sz, _vp], ctypes.c_int),
     "pds_backward_f32": ([_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _i32, _i32, _i32, _tp, _vp, _vp, _vp, _vp,
                           _sz, _vp], ctypes.c_int),
+    "pds_bins_bytes": ([_i64, _i32, _i32, _i32, _i32, _i32, _tp], _sz),
+    "pds_forward_bins_f32": ([_vp, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _i32, _i32, _i32, _tp, _vp, _vp, _sz, _vp],
+                             ctypes.c_int),
+    "pds_backward_bins_f32": ([_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _i32, _i32, _i32, _tp, _vp, _vp, _vp,
+                               _vp, _sz, _vp], ctypes.c_int),
     "pds_last_error": ([], ctypes.c_char_p),
     "pds_version": ([], ctypes.c_char_p),
 }

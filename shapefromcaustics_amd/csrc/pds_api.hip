@@ -71,6 +71,11 @@ pds::PhotonArgs photon_args(const float* E, const float* xp, const float* M, con
     return P;
 }
 
+size_t bins_bytes(int64_t N, int32_t C, int32_t H, int32_t W, int32_t cap, int32_t mode, const pds_tuning* t) {
+    if (N < 0 || C <= 0 || H <= 0 || W <= 0 || mode == PDS_MODE_PLAIN) return 0;
+    return pds::binned_workspace_bytes(N, C, H, W, cap, mode, t, ellipse_trivial(cap, W, H), /*keep=*/true);
+}
+
 pds::ImageArgs image_args(int32_t C, int32_t H, int32_t W, int32_t cap) {
     pds::ImageArgs I;
     I.C = C;
@@ -87,33 +92,21 @@ pds::ImageArgs image_args(int32_t C, int32_t H, int32_t W, int32_t cap) {
 
 }  // namespace
 
-extern "C" {
+namespace {
 
-const char* pds_last_error(void) { return g_last_error; }
-
-const char* pds_version(void) { return "pds_hip 0.1 gfx950"; }
-
-size_t pds_forward_workspace_bytes(int64_t N, int32_t C, int32_t H, int32_t W, int32_t max_pixel_radius,
-                                   int32_t mode, const pds_tuning* t) {
-    if (N < 0 || C <= 0 || H <= 0 || W <= 0 || mode == PDS_MODE_PLAIN) return 0;
-    return pds::binned_workspace_bytes(N, C, H, W, max_pixel_radius, mode, t,
-                                       ellipse_trivial(max_pixel_radius, W, H));
-}
-
-size_t pds_backward_workspace_bytes(int64_t, int32_t, int32_t, int32_t, int32_t, int32_t, const pds_tuning*) {
-    return 0;
-}
-
-int pds_forward_f32(const float* Ep, const float* xp, const float* Mp, const int64_t* cp, const float* radius,
-                    int64_t N, int32_t C, int32_t H, int32_t W, int32_t max_pixel_radius, int32_t mode,
-                    const pds_tuning* t, float* out, void* workspace, size_t workspace_bytes, void* stream) {
+int forward_impl(const float* Ep, const float* xp, const float* Mp, const int64_t* cp, const float* radius,
+                 int64_t N, int32_t C, int32_t H, int32_t W, int32_t max_pixel_radius, int32_t mode,
+                 const pds_tuning* t, float* out, void* workspace, size_t workspace_bytes, bool keep, void* stream) {
     g_last_error[0] = '\0';
     int rc = check_common(N, C, H, W, mode, Ep, xp, Mp, cp, radius);
     if (rc) return rc;
     const int64_t numel = (int64_t)C * H * W;
     if (numel == 0) return PDS_OK;
     if (!out) return fail(PDS_ERR_ARGUMENT, "null output pointer");
-    const size_t need = pds_forward_workspace_bytes(N, C, H, W, max_pixel_radius, mode, t);
+    if (keep && mode == PDS_MODE_PLAIN) return fail(PDS_ERR_ARGUMENT, "the plain mode keeps no bins");
+    const size_t need = keep ? bins_bytes(N, C, H, W, max_pixel_radius, mode, t)
+                             : pds_forward_workspace_bytes(N, C, H, W, max_pixel_radius, mode, t);
+    if (keep && need == 0) return fail(PDS_ERR_ARGUMENT, "this problem cannot be binned: use pds_forward_f32");
     if (need > 0 && (workspace == nullptr || workspace_bytes < need))
         return fail(PDS_ERR_WORKSPACE, "workspace of %zu bytes is smaller than the %zu bytes required",
                     workspace ? workspace_bytes : (size_t)0, need);
@@ -126,20 +119,17 @@ int pds_forward_f32(const float* Ep, const float* xp, const float* Mp, const int
         if ((e = pds::launch_forward_plain(P, I, out, s)) != hipSuccess) return hip_fail(e, "plain forward");
         return PDS_OK;
     }
-    rc = pds::binned_forward(P, I, mode, t, out, workspace, workspace_bytes, s);
+    rc = pds::binned_forward(P, I, mode, t, out, workspace, workspace_bytes, keep, s);
     if (rc == PDS_ERR_HIP) return hip_fail(hipGetLastError(), "binned forward");
     if (rc) return fail(rc, "binned forward: %s", pds::binned_last_error());
     return PDS_OK;
 }
 
-int pds_backward_f32(const float* grad, const float* Ep, const float* xp, const float* Mp, const int64_t* cp,
-                     const float* radius, int64_t N, int32_t C, int32_t H, int32_t W, int32_t max_pixel_radius,
-                     int32_t mode, const pds_tuning* t, float* grad_Ep, float* grad_xp, float* grad_Mp,
-                     void* workspace, size_t workspace_bytes, void* stream) {
+int backward_impl(const float* grad, const float* Ep, const float* xp, const float* Mp, const int64_t* cp,
+                  const float* radius, int64_t N, int32_t C, int32_t H, int32_t W, int32_t max_pixel_radius,
+                  int32_t mode, const pds_tuning* t, float* grad_Ep, float* grad_xp, float* grad_Mp, void* workspace,
+                  size_t workspace_bytes, bool reuse, void* stream) {
     g_last_error[0] = '\0';
-    (void)t;
-    (void)workspace;
-    (void)workspace_bytes;
     int rc = check_common(N, C, H, W, mode, Ep, xp, Mp, cp, radius);
     if (rc) return rc;
     if (N == 0) return PDS_OK;
@@ -158,12 +148,78 @@ int pds_backward_f32(const float* grad, const float* Ep, const float* xp, const 
         if ((e = pds::launch_fill_zero(grad_Mp, 6 * N, s)) != hipSuccess) return hip_fail(e, "zero fill");
         return PDS_OK;
     }
+    const size_t need = bins_bytes(N, C, H, W, max_pixel_radius, mode, t);
+    if (reuse && need == 0) return fail(PDS_ERR_ARGUMENT, "this problem has no bins: use pds_backward_f32");
+    if (mode != PDS_MODE_PLAIN && need > 0) {
+        if (workspace == nullptr || workspace_bytes < need)
+            return fail(PDS_ERR_WORKSPACE, "workspace of %zu bytes is smaller than the %zu bytes required",
+                        workspace ? workspace_bytes : (size_t)0, need);
+        rc = pds::binned_backward(grad, P, I, mode, t, G, workspace, workspace_bytes, reuse, s);
+        if (rc == PDS_ERR_HIP) return hip_fail(hipGetLastError(), "binned backward");
+        if (rc) return fail(rc, "binned backward: %s", pds::binned_last_error());
+        return PDS_OK;
+    }
     if (mode == PDS_MODE_PLAIN || !I.ell_trivial) {
         if ((e = pds::launch_backward_plain(grad, P, I, G, s)) != hipSuccess) return hip_fail(e, "plain backward");
         return PDS_OK;
     }
     if ((e = pds::launch_backward_flat(grad, P, I, G, s)) != hipSuccess) return hip_fail(e, "backward");
     return PDS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pds_last_error(void) { return g_last_error; }
+
+const char* pds_version(void) { return "pds_hip 0.1 gfx950"; }
+
+size_t pds_forward_workspace_bytes(int64_t N, int32_t C, int32_t H, int32_t W, int32_t max_pixel_radius,
+                                   int32_t mode, const pds_tuning* t) {
+    if (N < 0 || C <= 0 || H <= 0 || W <= 0 || mode == PDS_MODE_PLAIN) return 0;
+    return pds::binned_workspace_bytes(N, C, H, W, max_pixel_radius, mode, t,
+                                       ellipse_trivial(max_pixel_radius, W, H), /*keep=*/false);
+}
+
+size_t pds_bins_bytes(int64_t N, int32_t C, int32_t H, int32_t W, int32_t max_pixel_radius, int32_t mode,
+                      const pds_tuning* t) {
+    return bins_bytes(N, C, H, W, max_pixel_radius, mode, t);
+}
+
+size_t pds_backward_workspace_bytes(int64_t N, int32_t C, int32_t H, int32_t W, int32_t max_pixel_radius,
+                                    int32_t mode, const pds_tuning* t) {
+    return pds_bins_bytes(N, C, H, W, max_pixel_radius, mode, t);
+}
+
+int pds_forward_f32(const float* Ep, const float* xp, const float* Mp, const int64_t* cp, const float* radius,
+                    int64_t N, int32_t C, int32_t H, int32_t W, int32_t max_pixel_radius, int32_t mode,
+                    const pds_tuning* t, float* out, void* workspace, size_t workspace_bytes, void* stream) {
+    return forward_impl(Ep, xp, Mp, cp, radius, N, C, H, W, max_pixel_radius, mode, t, out, workspace,
+                        workspace_bytes, false, stream);
+}
+
+int pds_forward_bins_f32(const float* Ep, const float* xp, const float* Mp, const int64_t* cp, const float* radius,
+                         int64_t N, int32_t C, int32_t H, int32_t W, int32_t max_pixel_radius, int32_t mode,
+                         const pds_tuning* t, float* out, void* bins, size_t bins_bytes, void* stream) {
+    return forward_impl(Ep, xp, Mp, cp, radius, N, C, H, W, max_pixel_radius, mode, t, out, bins, bins_bytes, true,
+                        stream);
+}
+
+int pds_backward_f32(const float* grad, const float* Ep, const float* xp, const float* Mp, const int64_t* cp,
+                     const float* radius, int64_t N, int32_t C, int32_t H, int32_t W, int32_t max_pixel_radius,
+                     int32_t mode, const pds_tuning* t, float* grad_Ep, float* grad_xp, float* grad_Mp,
+                     void* workspace, size_t workspace_bytes, void* stream) {
+    return backward_impl(grad, Ep, xp, Mp, cp, radius, N, C, H, W, max_pixel_radius, mode, t, grad_Ep, grad_xp,
+                         grad_Mp, workspace, workspace_bytes, false, stream);
+}
+
+int pds_backward_bins_f32(const float* grad, const float* Ep, const float* xp, const float* Mp, const int64_t* cp,
+                          const float* radius, int64_t N, int32_t C, int32_t H, int32_t W, int32_t max_pixel_radius,
+                          int32_t mode, const pds_tuning* t, float* grad_Ep, float* grad_xp, float* grad_Mp,
+                          void* bins, size_t bins_bytes, void* stream) {
+    return backward_impl(grad, Ep, xp, Mp, cp, radius, N, C, H, W, max_pixel_radius, mode, t, grad_Ep, grad_xp,
+                         grad_Mp, bins, bins_bytes, true, stream);
 }
 
 }  // extern "C"

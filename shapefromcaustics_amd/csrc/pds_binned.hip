@@ -46,6 +46,8 @@ constexpr int kTileChunk = 2;
 constexpr int kDefaultTile = 32;
 constexpr int kDefaultSeg = 2048;
 constexpr double kDefaultCapacity = 2.0;  // records per photon reserved in the workspace
+constexpr int kMaxRecsPerPhoton = 4;      // backward: photons with more records are recomputed directly
+constexpr uint8_t kSpecial = 255;         // class of photons the backward recomputes directly
 
 thread_local const char* g_err = "";
 
@@ -64,7 +66,7 @@ struct Geom {
 };
 
 struct Layout {
-    size_t hist, totals, binoff, segs, ctrl, recs, acc64, end;
+    size_t hist, totals, binoff, segs, ctrl, recs, slots, info, partials, acc64, end;
 };
 
 inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -108,7 +110,9 @@ bool make_geom(int64_t N, int32_t C, int32_t H, int32_t W, int32_t cap, int32_t 
     return true;
 }
 
-Layout make_layout(const Geom& g, int32_t C, int32_t H, int32_t W) {
+// keep: also the per-photon record slots and classes and the per-record
+// partial gradients the backward needs.
+Layout make_layout(const Geom& g, int64_t N, int32_t C, int32_t H, int32_t W, bool keep) {
     Layout L;
     size_t o = 0;
     L.hist = o;
@@ -123,6 +127,14 @@ Layout make_layout(const Geom& g, int32_t C, int32_t H, int32_t W) {
     o = align256(o + sizeof(Ctrl));
     L.recs = o;
     o = align256(o + 32 * (size_t)g.cap_records);
+    L.slots = L.info = L.partials = o;
+    if (keep) {
+        o = align256(o + sizeof(uint32_t) * kMaxRecsPerPhoton * (size_t)N);
+        L.info = o;
+        o = align256(o + (size_t)N);
+        L.partials = o;
+        o = align256(o + 32 * (size_t)g.cap_records);
+    }
     L.acc64 = o;
     if (g.det) o = align256(o + sizeof(uint64_t) * (size_t)C * H * W);
     L.end = o;
@@ -133,7 +145,8 @@ Layout make_layout(const Geom& g, int32_t C, int32_t H, int32_t W) {
 // photon -> clipped footprint box (forward semantics)
 // ---------------------------------------------------------------------------
 struct BinPhoton {
-    bool ok;
+    bool rec;      // has records: finite frame and a non-empty footprint box
+    bool special;  // the backward recomputes it directly: non-finite energy or frame
     int32_t c;
     int32_t x0, x1, y0, y1;  // box on the image, inclusive
     Frame f;
@@ -141,52 +154,51 @@ struct BinPhoton {
     int32_t rx, ry, ox, oy;  // window radii and pxc - ix + 1, pyc - iy + 1
 };
 
-// Photons that can add to the image: channel in range, E > 0 (the reference's
-// val > 0 filter, photon_differentials.cu:106), finite frame.  Their box is
-// window (:80-96) n image n footprint bounding box.
+// Photon i's footprint box: window (photon_differentials.cu:80-96) n image n
+// ellipse bounding box.  Photons outside the channel range or with an empty
+// window contribute nothing and have zero gradients (neither rec nor special).
+// A non-finite centre, matrix or M[:,2] makes q NaN or infinite, so K = 0 and
+// the forward gets nothing from it, but the reference's backward turns it
+// into NaN gradients: such photons, and those with a non-finite energy, are
+// "special" and the backward evaluates them by a direct serial walk.  Photons
+// with E <= 0 get records (their gE is sum K g); the forward's val > 0 test
+// (:106) skips their pixels.
 __device__ __forceinline__ BinPhoton bin_photon(const PhotonArgs& P, const ImageArgs& I, int64_t i) {
     BinPhoton b;
-    b.ok = false;
+    b.rec = false;
+    b.special = false;
     const int64_t c = P.cp[i];
     if (c < 0 || c >= I.C) return b;
     const int64_t N = P.N;
     const float e = P.E[i];
-    if (!(e > 0.0f)) return b;
     const float cx = P.xp[i], cy = P.xp[N + i];
     const Window win = photon_window(P.radius[i], cx, cy, I.W, I.H, I.cap);
     if (win.rx < 0 || win.ry < 0) return b;
-    int64_t x0 = max((int64_t)win.pxc - win.rx, (int64_t)0);
-    int64_t x1 = min((int64_t)win.pxc + win.rx, (int64_t)I.W - 1);
-    int64_t y0 = max((int64_t)win.pyc - win.ry, (int64_t)0);
-    int64_t y1 = min((int64_t)win.pyc + win.ry, (int64_t)I.H - 1);
+    int32_t x0 = (int32_t)max((int64_t)win.pxc - win.rx, (int64_t)0);
+    int32_t x1 = (int32_t)min((int64_t)win.pxc + win.rx, (int64_t)I.W - 1);
+    int32_t y0 = (int32_t)max((int64_t)win.pyc - win.ry, (int64_t)0);
+    int32_t y1 = (int32_t)min((int64_t)win.pyc + win.ry, (int64_t)I.H - 1);
     if (x0 > x1 || y0 > y1) return b;
     const float m00 = P.M[i], m01 = P.M[N + i], m10 = P.M[3 * N + i], m11 = P.M[4 * N + i];
     const float poison = P.M[2 * N + i] * 0.0f + P.M[5 * N + i] * 0.0f;
-    // a non-finite centre, matrix or M[:,2] makes q NaN or infinite: K = 0, no contribution
-    if (!(isfinite(cx) && isfinite(cy) && isfinite(m00) && isfinite(m01) && isfinite(m10) && isfinite(m11) &&
-          poison == 0.0f))
-        return b;
+    const bool finite_frame = isfinite(cx) && isfinite(cy) && isfinite(m00) && isfinite(m01) && isfinite(m10) &&
+                              isfinite(m11) && poison == 0.0f;
+    b.special = !finite_frame || !isfinite(e);
+    if (!finite_frame) return b;
     b.f = photon_frame(cx, cy, m00, m01, m10, m11, 0.0f, I.W, I.H, I.inv_hw, I.inv_hh);
-    {
-        int32_t a0 = (int32_t)x0, a1 = (int32_t)x1, c0 = (int32_t)y0, c1 = (int32_t)y1;
-        clip_to_ellipse(b.f, a0, a1, c0, c1);
-        if (a0 > a1 || c0 > c1) return b;
-        x0 = a0;
-        x1 = a1;
-        y0 = c0;
-        y1 = c1;
-    }
-    b.x0 = (int32_t)x0;
-    b.x1 = (int32_t)x1;
-    b.y0 = (int32_t)y0;
-    b.y1 = (int32_t)y1;
+    clip_to_ellipse(b.f, x0, x1, y0, y1);
+    if (x0 > x1 || y0 > y1) return b;
+    b.x0 = x0;
+    b.x1 = x1;
+    b.y0 = y0;
+    b.y1 = y1;
     b.c = (int32_t)c;
     b.e = e;
     b.rx = win.rx;
     b.ry = win.ry;
     b.ox = win.pxc - b.f.ix + 1;  // in {0, 1, 2}: trunc vs round of the same double
     b.oy = win.pyc - b.f.iy + 1;
-    b.ok = true;
+    b.rec = true;
     return b;
 }
 
@@ -214,8 +226,8 @@ __global__ __launch_bounds__(kBinThreads) void bin_count_kernel(PhotonArgs P, Im
     float emax = 0.0f;
     for (int64_t i = lo + threadIdx.x; i < hi; i += kBinThreads) {
         const BinPhoton b = bin_photon(P, I, i);
-        if (!b.ok) continue;
-        if (isfinite(b.e)) emax = fmaxf(emax, b.e);
+        if (!b.rec) continue;
+        if (b.e > 0.0f && isfinite(b.e)) emax = fmaxf(emax, b.e);
         const int tx0 = b.x0 >> g.tshift, tx1 = b.x1 >> g.tshift;
         const int ty0 = b.y0 >> g.tshift, ty1 = b.y1 >> g.tshift;
         for (int ty = ty0; ty <= ty1; ++ty)
@@ -355,9 +367,11 @@ __global__ __launch_bounds__(1024) void bin_plan_kernel(Geom g, int64_t N, const
 // ---------------------------------------------------------------------------
 // 5. scatter
 // ---------------------------------------------------------------------------
+template <bool KEEP>
 __global__ __launch_bounds__(kBinThreads) void bin_scatter_kernel(PhotonArgs P, ImageArgs I, Geom g,
                                                                   const uint32_t* __restrict__ hist,
-                                                                  uint4* __restrict__ recs, const Ctrl* ctrl) {
+                                                                  uint4* __restrict__ recs, uint32_t* __restrict__ slots,
+                                                                  uint8_t* __restrict__ info, const Ctrl* ctrl) {
     extern __shared__ uint32_t cur[];  // nbins entries
     if (ctrl->overflow) return;
     const uint32_t* row = hist + (size_t)blockIdx.x * g.nbins;
@@ -368,22 +382,27 @@ __global__ __launch_bounds__(kBinThreads) void bin_scatter_kernel(PhotonArgs P, 
     const int half = g.T >> 1;
     for (int64_t i = lo + threadIdx.x; i < hi; i += kBinThreads) {
         const BinPhoton b = bin_photon(P, I, i);
-        if (!b.ok) continue;
-        const uint4 r0 = make_uint4(__float_as_uint(b.f.fx), __float_as_uint(b.f.fy), __float_as_uint(b.e), 0u);
-        const uint4 r1 = make_uint4(__float_as_uint(b.f.a00), __float_as_uint(b.f.a01), __float_as_uint(b.f.a10),
-                                    __float_as_uint(b.f.a11));
-        const int tx0 = b.x0 >> g.tshift, tx1 = b.x1 >> g.tshift;
-        const int ty0 = b.y0 >> g.tshift, ty1 = b.y1 >> g.tshift;
-        for (int ty = ty0; ty <= ty1; ++ty) {
-            for (int tx = tx0; tx <= tx1; ++tx) {
-                const uint32_t slot = atomicAdd(&cur[(b.c * g.tiles_y + ty) * g.tiles_x + tx], 1u);
-                uint4 q = r0;
-                q.w = pack_rec(b.f.ix - ((tx << g.tshift) + half), b.f.iy - ((ty << g.tshift) + half), b.rx, b.ry,
-                               b.ox, b.oy);
-                recs[2 * (size_t)slot] = q;
-                recs[2 * (size_t)slot + 1] = r1;
+        int n = 0;
+        if (b.rec) {
+            const uint4 r0 = make_uint4(__float_as_uint(b.f.fx), __float_as_uint(b.f.fy), __float_as_uint(b.e), 0u);
+            const uint4 r1 = make_uint4(__float_as_uint(b.f.a00), __float_as_uint(b.f.a01), __float_as_uint(b.f.a10),
+                                        __float_as_uint(b.f.a11));
+            const int tx0 = b.x0 >> g.tshift, tx1 = b.x1 >> g.tshift;
+            const int ty0 = b.y0 >> g.tshift, ty1 = b.y1 >> g.tshift;
+            for (int ty = ty0; ty <= ty1; ++ty) {
+                for (int tx = tx0; tx <= tx1; ++tx) {
+                    const uint32_t slot = atomicAdd(&cur[(b.c * g.tiles_y + ty) * g.tiles_x + tx], 1u);
+                    uint4 q = r0;
+                    q.w = pack_rec(b.f.ix - ((tx << g.tshift) + half), b.f.iy - ((ty << g.tshift) + half), b.rx,
+                                   b.ry, b.ox, b.oy);
+                    recs[2 * (size_t)slot] = q;
+                    recs[2 * (size_t)slot + 1] = r1;
+                    if (KEEP && n < kMaxRecsPerPhoton) slots[(size_t)n * P.N + i] = slot;
+                    ++n;
+                }
             }
         }
+        if (KEEP) info[i] = (b.special || n > kMaxRecsPerPhoton) ? kSpecial : (uint8_t)n;
     }
 }
 
@@ -422,6 +441,41 @@ __device__ __forceinline__ float from_fixed(unsigned long long v, int e) {
     return (float)ldexp((double)(long long)v, e);
 }
 
+// Decode record j for the tile with origin (X0, Y0) and last pixel (X1, Y1)
+// into a flattened-walk record: box = window n tile n ellipse bounding box,
+// LDS tile index of its origin in r.base.
+__device__ __forceinline__ void decode_record(const uint4* __restrict__ recs, size_t j, int X0, int Y0, int X1, int Y1,
+                                              int half, int T, FlatRec& r) {
+    const uint4 q0 = recs[2 * j], q1 = recs[2 * j + 1];
+    const uint32_t pk = q0.w;
+    const int ix = X0 + half + (int)(int8_t)(pk & 0xff);
+    const int iy = Y0 + half + (int)(int8_t)((pk >> 8) & 0xff);
+    const int rx = (pk >> 16) & 63, ry = (pk >> 22) & 63;
+    const int pxc = ix + (int)((pk >> 28) & 3) - 1, pyc = iy + (int)(pk >> 30) - 1;
+    r.f.ix = ix;
+    r.f.iy = iy;
+    r.f.fx = __uint_as_float(q0.x);
+    r.f.fy = __uint_as_float(q0.y);
+    r.e = __uint_as_float(q0.z);
+    r.f.a00 = __uint_as_float(q1.x);
+    r.f.a01 = __uint_as_float(q1.y);
+    r.f.a10 = __uint_as_float(q1.z);
+    r.f.a11 = __uint_as_float(q1.w);
+    int x0 = max(max(pxc - rx, X0), 0), x1 = min(pxc + rx, X1);
+    int y0 = max(max(pyc - ry, Y0), 0), y1 = min(pyc + ry, Y1);
+    clip_to_ellipse(r.f, x0, x1, y0, y1);
+    if (x0 <= x1 && y0 <= y1) {
+        const int bw = x1 - x0 + 1, bh = y1 - y0 + 1;
+        r.cpr = (bw + kTileChunk - 1) / kTileChunk;
+        r.inv_cpr = 1.0f / (float)r.cpr;
+        r.nitems = bh * r.cpr;
+        r.kx0 = x0 - ix;
+        r.ky0 = y0 - iy;
+        r.bw = bw;
+        r.base = (y0 - Y0) * T + (x0 - X0);  // index into the LDS tile
+    }
+}
+
 template <bool DET>
 __global__ __launch_bounds__(kTileThreads) void bin_tile_kernel(Geom g, ImageArgs I, const int4* __restrict__ segs,
                                                                 const uint4* __restrict__ recs, const Ctrl* ctrl,
@@ -444,6 +498,8 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_kernel(Geom g, ImageArg
     const int X1 = min(X0 + T, I.W) - 1, Y1 = min(Y0 + T, I.H) - 1;
     const int half = T >> 1;
     __syncthreads();
+    ImageArgs TI = I;
+    TI.W = T;  // walk_items computes base + row * W + col: LDS tile row pitch
 
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     WaveRecs& R = wr[w];
@@ -456,40 +512,8 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_kernel(Geom g, ImageArg
         r.inv_cpr = 1.0f;
         r.e = 0.f;
         r.f.fx = r.f.fy = r.f.a00 = r.f.a01 = r.f.a10 = r.f.a11 = 0.f;
-        if (b0 + lane < d.z) {
-            const size_t j = (size_t)d.y + b0 + lane;
-            const uint4 q0 = recs[2 * j], q1 = recs[2 * j + 1];
-            const uint32_t pk = q0.w;
-            const int ix = X0 + half + (int)(int8_t)(pk & 0xff);
-            const int iy = Y0 + half + (int)(int8_t)((pk >> 8) & 0xff);
-            const int rx = (pk >> 16) & 63, ry = (pk >> 22) & 63;
-            const int pxc = ix + (int)((pk >> 28) & 3) - 1, pyc = iy + (int)(pk >> 30) - 1;
-            r.f.fx = __uint_as_float(q0.x);
-            r.f.fy = __uint_as_float(q0.y);
-            r.e = __uint_as_float(q0.z);
-            r.f.a00 = __uint_as_float(q1.x);
-            r.f.a01 = __uint_as_float(q1.y);
-            r.f.a10 = __uint_as_float(q1.z);
-            r.f.a11 = __uint_as_float(q1.w);
-            int x0 = max(max(pxc - rx, X0), 0), x1 = min(pxc + rx, X1);
-            int y0 = max(max(pyc - ry, Y0), 0), y1 = min(pyc + ry, Y1);
-            r.f.ix = ix;
-            r.f.iy = iy;
-            clip_to_ellipse(r.f, x0, x1, y0, y1);
-            if (x0 <= x1 && y0 <= y1) {
-                const int bw = x1 - x0 + 1, bh = y1 - y0 + 1;
-                r.cpr = (bw + kTileChunk - 1) / kTileChunk;
-                r.inv_cpr = 1.0f / (float)r.cpr;
-                r.nitems = bh * r.cpr;
-                r.kx0 = x0 - ix;
-                r.ky0 = y0 - iy;
-                r.bw = bw;
-                r.base = (y0 - Y0) * T + (x0 - X0);  // index into the LDS tile
-            }
-        }
+        if (b0 + lane < d.z) decode_record(recs, (size_t)d.y + b0 + lane, X0, Y0, X1, Y1, half, T, r);
         const int64_t total = publish_recs(R, r, lane);
-        ImageArgs TI = I;
-        TI.W = T;  // walk_items computes base + row * W + col: LDS tile row pitch
         walk_items<kTileChunk>(R, r, TI, lane, total, [&](const FlatItem& it) {
 #pragma unroll
             for (int k = 0; k < kTileChunk; ++k) {
@@ -557,6 +581,139 @@ __global__ __launch_bounds__(kTileThreads) void bin_convert_split_kernel(Geom g,
             const int64_t o = ((int64_t)c * I.H + py) * I.W + px;
             out[o] = from_fixed(acc64[o], fe);
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// backward 1: tile adjoint.  The segment's gradient tile is staged in LDS; each
+// record's seven sums over its pixels in the tile are reduced in LDS and
+// stored, in the reference's units, at the record's slot (coalesced).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kTileThreads) void bin_tile_bwd_kernel(Geom g, ImageArgs I, const int4* __restrict__ segs,
+                                                                    const uint4* __restrict__ recs, const Ctrl* ctrl,
+                                                                    const float* __restrict__ grad,
+                                                                    float4* __restrict__ partials) {
+    extern __shared__ __align__(16) float gt[];  // T * T upstream gradient tile
+    __shared__ WaveRecs wr[kTileThreads / 64];
+    __shared__ float acc[kTileThreads / 64][7][64];
+    const uint32_t s = blockIdx.x;
+    if (s >= ctrl->nseg) return;
+    const int4 d = segs[s];
+    const int T = g.T, TT = T * T;
+    const int tpc = g.tiles_x * g.tiles_y;
+    const int c = d.x / tpc, tt = d.x - c * tpc;
+    const int ty = tt / g.tiles_x, tx = tt - ty * g.tiles_x;
+    const int X0 = tx << g.tshift, Y0 = ty << g.tshift;
+    const int X1 = min(X0 + T, I.W) - 1, Y1 = min(Y0 + T, I.H) - 1;
+    const int half = T >> 1;
+    for (int k = threadIdx.x; k < TT; k += kTileThreads) {
+        const int px = X0 + (k & (T - 1)), py = Y0 + (k >> g.tshift);
+        gt[k] = (px < I.W && py < I.H) ? grad[((int64_t)c * I.H + py) * I.W + px] : 0.0f;
+    }
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    WaveRecs& R = wr[w];
+    R.mark[lane] = -1;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) acc[w][k][lane] = 0.f;
+    __syncthreads();
+    const float hw = 0.5f * I.fw, hh = 0.5f * I.fh, iw = 2.0f / I.fw, ih = 2.0f / I.fh;
+    ImageArgs TI = I;
+    TI.W = T;  // walk_items: LDS tile row pitch
+    for (int b0 = w * 64; b0 < d.z; b0 += kTileThreads) {
+        FlatRec r;
+        r.nitems = 0;
+        r.kx0 = r.ky0 = r.bw = r.base = 0;
+        r.cpr = 1;
+        r.inv_cpr = 1.0f;
+        r.e = 0.f;
+        r.f.fx = r.f.fy = r.f.a00 = r.f.a01 = r.f.a10 = r.f.a11 = 0.f;
+        const size_t j = (size_t)d.y + b0 + lane;
+        if (b0 + lane < d.z) decode_record(recs, j, X0, Y0, X1, Y1, half, T, r);
+        const int64_t total = publish_recs(R, r, lane);
+        walk_items<kTileChunk>(R, r, TI, lane, total, [&](const FlatItem& it) {
+            float a[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int k = 0; k < kTileChunk; ++k) {
+                if (k < it.n) {
+                    const float dx = (float)(it.kx + k) - it.f.fx;
+                    const float u = it.f.a00 * dx + it.f.a01 * it.dy;
+                    const float v = it.f.a10 * dx + it.f.a11 * it.dy;
+                    const float q = u * u + v * v;
+                    const float gg = gt[it.pix + k];
+                    const float sc = dk_over_r(q, it.e * gg);
+                    accumulate_terms(a, q, gg, sc, u, v, dx, it.dy, it.f);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 7; ++k) atomicAdd(&acc[w][k][it.rec], a[k]);
+        });
+        wave_lds_fence();
+        float t7[7];
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            t7[k] = acc[w][k][lane];
+            acc[w][k][lane] = 0.f;
+        }
+        if (b0 + lane < d.z) {
+            partials[2 * j] = make_float4(t7[0], t7[1] * hw, t7[2] * hh, t7[3] * iw);
+            partials[2 * j + 1] = make_float4(t7[4] * ih, t7[5] * iw, t7[6] * ih, 0.0f);
+        }
+        wave_lds_fence();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// backward 2: finalize in photon order -- sum each photon's record partials
+// (in tile order), or walk a special photon directly; coalesced stores.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void bin_finalize_kernel(const float* __restrict__ grad, PhotonArgs P, ImageArgs I,
+                                                           const Ctrl* ctrl, const uint8_t* __restrict__ info,
+                                                           const uint32_t* __restrict__ slots,
+                                                           const float4* __restrict__ partials, GradArgs G) {
+    if (ctrl->overflow) return;
+    const int64_t N = P.N;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride) {
+        const uint8_t cls = info[i];
+        if (cls == kSpecial) {
+            float a[7];
+            photon_grad_direct(grad, P, I, i, /*clip_box=*/true, a);
+            store_grads(G, I, N, i, a);
+            continue;
+        }
+        float4 s0 = make_float4(0.f, 0.f, 0.f, 0.f), s1 = s0;
+        for (int k = 0; k < (int)cls; ++k) {
+            const size_t j = slots[(size_t)k * N + i];
+            const float4 p0 = partials[2 * j], p1 = partials[2 * j + 1];
+            s0.x += p0.x;
+            s0.y += p0.y;
+            s0.z += p0.z;
+            s0.w += p0.w;
+            s1.x += p1.x;
+            s1.y += p1.y;
+            s1.z += p1.z;
+        }
+        G.gE[i] = s0.x;
+        G.gx[i] = s0.y;
+        G.gx[N + i] = s0.z;
+        G.gM[i] = s0.w;
+        G.gM[N + i] = s1.x;
+        G.gM[2 * N + i] = 0.0f;
+        G.gM[3 * N + i] = s1.y;
+        G.gM[4 * N + i] = s1.z;
+        G.gM[5 * N + i] = 0.0f;
+    }
+}
+
+// capacity overflow: every photon by the direct walk
+__global__ __launch_bounds__(256) void guarded_bwd_direct_kernel(const float* __restrict__ grad, PhotonArgs P,
+                                                                 ImageArgs I, const Ctrl* ctrl, GradArgs G) {
+    if (!ctrl->overflow) return;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < P.N; i += stride) {
+        float a[7];
+        photon_grad_direct(grad, P, I, i, /*clip_box=*/true, a);
+        store_grads(G, I, P.N, i, a);
     }
 }
 
@@ -693,111 +850,199 @@ __global__ void ctrl_init_kernel(Ctrl* ctrl) {
 
 const char* binned_last_error() { return g_err; }
 
-size_t binned_workspace_bytes(int64_t N, int32_t C, int32_t H, int32_t W, int32_t cap, int32_t mode,
-                              const pds_tuning* t, bool ell_trivial) {
-    if (mode == PDS_MODE_PLAIN) return 0;
-    const bool det = mode == PDS_MODE_DETERMINISTIC;
-    Geom g;
-    if (!ell_trivial || !make_geom(N, C, H, W, cap, mode, t, g)) {
-        // photon-order paths: only the deterministic one needs the int64 image and a control block
-        if (!det) return 0;
-        return align256(sizeof(Ctrl)) + sizeof(uint64_t) * (size_t)C * H * W;
-    }
-    return make_layout(g, C, H, W).end;
+namespace {
+
+struct Buffers {
+    uint32_t *hist, *totals, *binoff, *slots;
+    int4* segs;
+    Ctrl* ctrl;
+    uint4* recs;
+    uint8_t* info;
+    float4* partials;
+    unsigned long long* acc64;
+};
+
+Buffers buffers(void* ws, const Layout& L) {
+    unsigned char* b = (unsigned char*)ws;
+    Buffers B;
+    B.hist = (uint32_t*)(b + L.hist);
+    B.totals = (uint32_t*)(b + L.totals);
+    B.binoff = (uint32_t*)(b + L.binoff);
+    B.segs = (int4*)(b + L.segs);
+    B.ctrl = (Ctrl*)(b + L.ctrl);
+    B.recs = (uint4*)(b + L.recs);
+    B.slots = (uint32_t*)(b + L.slots);
+    B.info = (uint8_t*)(b + L.info);
+    B.partials = (float4*)(b + L.partials);
+    B.acc64 = (unsigned long long*)(b + L.acc64);
+    return B;
 }
 
-int binned_forward(const PhotonArgs& P, const ImageArgs& I, int32_t mode, const pds_tuning* t, float* out, void* ws,
-                   size_t ws_bytes, hipStream_t s) {
+// Can this problem take the binned path?  (window ellipse test inactive, cap
+// <= 63, bins fit the LDS histogram, 32-bit record offsets)
+bool binnable(const ImageArgs& I, int64_t N, int32_t mode, const pds_tuning* t, Geom& g) {
+    return I.ell_trivial && make_geom(N, I.C, I.H, I.W, I.cap, mode, t, g);
+}
+
+// count -> columns -> plan -> offsets -> scatter
+void launch_build(const PhotonArgs& P, const ImageArgs& I, const Geom& g, const Buffers& B, bool keep,
+                  hipStream_t s) {
+    const unsigned colblocks = (unsigned)((g.nbins + 63) / 64);
+    const size_t hist_lds = sizeof(uint32_t) * (size_t)g.nbins;
+    hipLaunchKernelGGL(ctrl_init_kernel, dim3(1), dim3(1), 0, s, B.ctrl);
+    hipLaunchKernelGGL(bin_count_kernel, dim3(g.G), dim3(kBinThreads), hist_lds, s, P, I, g, B.hist, B.ctrl);
+    hipLaunchKernelGGL(bin_columns_kernel<false>, dim3(colblocks), dim3(kColThreads), 0, s, g, B.hist, B.totals,
+                       (const uint32_t*)B.binoff, (const Ctrl*)B.ctrl);
+    hipLaunchKernelGGL(bin_plan_kernel, dim3(1), dim3(1024), 0, s, g, P.N, (const uint32_t*)B.totals, B.binoff,
+                       B.segs, B.ctrl);
+    hipLaunchKernelGGL(bin_columns_kernel<true>, dim3(colblocks), dim3(kColThreads), 0, s, g, B.hist, B.totals,
+                       (const uint32_t*)B.binoff, (const Ctrl*)B.ctrl);
+    if (keep)
+        hipLaunchKernelGGL(bin_scatter_kernel<true>, dim3(g.G), dim3(kBinThreads), hist_lds, s, P, I, g,
+                           (const uint32_t*)B.hist, B.recs, B.slots, B.info, (const Ctrl*)B.ctrl);
+    else
+        hipLaunchKernelGGL(bin_scatter_kernel<false>, dim3(g.G), dim3(kBinThreads), hist_lds, s, P, I, g,
+                           (const uint32_t*)B.hist, B.recs, (uint32_t*)nullptr, (uint8_t*)nullptr,
+                           (const Ctrl*)B.ctrl);
+}
+
+// zero split tiles -> tiles -> (convert) ; photon-order path on overflow
+void launch_forward_tiles(const PhotonArgs& P, const ImageArgs& I, const Geom& g, const Buffers& B, float* out,
+                          hipStream_t s) {
     const int64_t numel = (int64_t)I.C * I.H * I.W;
-    const bool det = mode == PDS_MODE_DETERMINISTIC;
-    Geom g;
-    if (!I.ell_trivial || !make_geom(P.N, I.C, I.H, I.W, I.cap, mode, t, g)) {
-        // no binning possible (tiny image with the window ellipse test active,
-        // cap > 63, or too many bins): photon-order paths
-        if (!det) {
-            if (launch_fill_zero(out, numel, s) != hipSuccess) return PDS_ERR_HIP;
-            if (!I.ell_trivial) return launch_forward_plain(P, I, out, s) == hipSuccess ? PDS_OK : PDS_ERR_HIP;
-            if (launch_forward_flat(P, I, out, s) != hipSuccess) return PDS_ERR_HIP;
-            return PDS_OK;
-        }
-        // deterministic: int64 fixed point over the whole image, photon order
-        if (ws == nullptr || ws_bytes < align256(sizeof(Ctrl)) + sizeof(uint64_t) * (size_t)numel) {
-            g_err = "workspace too small";
-            return PDS_ERR_WORKSPACE;
-        }
-        unsigned char* base = (unsigned char*)ws;
-        Ctrl* ctrl = (Ctrl*)base;
-        unsigned long long* acc = (unsigned long long*)(base + align256(sizeof(Ctrl)));
-        hipLaunchKernelGGL(ctrl_init_kernel, dim3(1), dim3(1), 0, s, ctrl);
-        hipLaunchKernelGGL(emax_kernel, dim3(1024), dim3(256), 0, s, P, I, ctrl);
-        hipLaunchKernelGGL(fixed_point_kernel, dim3(1), dim3(1), 0, s, ctrl, P.N);
-        hipLaunchKernelGGL(guarded_fill64_kernel, dim3(1024), dim3(256), 0, s, (const Ctrl*)ctrl, acc, numel);
-        if (I.ell_trivial)
-            hipLaunchKernelGGL(guarded_flat_kernel<true>, dim3(1024), dim3(256), 0, s, P, I, (const Ctrl*)ctrl, out,
-                               acc);
-        else if (P.N > 0)
-            hipLaunchKernelGGL(fwd_plain_fixed_kernel, dim3((unsigned)((P.N + 255) / 256)), dim3(256), 0, s, P, I,
-                               (const Ctrl*)ctrl, acc);
-        hipLaunchKernelGGL(guarded_convert_kernel, dim3(1024), dim3(256), 0, s, (const Ctrl*)ctrl,
-                           (const unsigned long long*)acc, out, numel);
-        return hipGetLastError() == hipSuccess ? PDS_OK : PDS_ERR_HIP;
+    const unsigned nsegs = (unsigned)g.max_segments;
+    const size_t tile_lds = (size_t)g.T * g.T * (g.det ? 8 : 4);
+    const unsigned fb = 1024;
+    if (g.det) {
+        hipLaunchKernelGGL(bin_zero_split_kernel<unsigned long long>, dim3(nsegs), dim3(kTileThreads), 0, s, g, I,
+                           (const int4*)B.segs, (const Ctrl*)B.ctrl, B.acc64);
+        hipLaunchKernelGGL(bin_tile_kernel<true>, dim3(nsegs), dim3(kTileThreads), tile_lds, s, g, I,
+                           (const int4*)B.segs, (const uint4*)B.recs, (const Ctrl*)B.ctrl, out, B.acc64);
+        hipLaunchKernelGGL(bin_convert_split_kernel, dim3(nsegs), dim3(kTileThreads), 0, s, g, I,
+                           (const int4*)B.segs, (const Ctrl*)B.ctrl, (const unsigned long long*)B.acc64, out);
+        hipLaunchKernelGGL(guarded_fill64_kernel, dim3(fb), dim3(256), 0, s, (const Ctrl*)B.ctrl, B.acc64, numel);
+        hipLaunchKernelGGL(guarded_flat_kernel<true>, dim3(fb), dim3(256), 0, s, P, I, (const Ctrl*)B.ctrl, out,
+                           B.acc64);
+        hipLaunchKernelGGL(guarded_convert_kernel, dim3(fb), dim3(256), 0, s, (const Ctrl*)B.ctrl,
+                           (const unsigned long long*)B.acc64, out, numel);
+    } else {
+        hipLaunchKernelGGL(bin_zero_split_kernel<float>, dim3(nsegs), dim3(kTileThreads), 0, s, g, I,
+                           (const int4*)B.segs, (const Ctrl*)B.ctrl, out);
+        hipLaunchKernelGGL(bin_tile_kernel<false>, dim3(nsegs), dim3(kTileThreads), tile_lds, s, g, I,
+                           (const int4*)B.segs, (const uint4*)B.recs, (const Ctrl*)B.ctrl, out,
+                           (unsigned long long*)nullptr);
+        hipLaunchKernelGGL(guarded_fill_kernel, dim3(fb), dim3(256), 0, s, (const Ctrl*)B.ctrl, out, numel);
+        hipLaunchKernelGGL(guarded_flat_kernel<false>, dim3(fb), dim3(256), 0, s, P, I, (const Ctrl*)B.ctrl, out,
+                           (unsigned long long*)nullptr);
     }
-    const Layout L = make_layout(g, I.C, I.H, I.W);
-    if (ws_bytes < L.end || ws == nullptr) {
+}
+
+// tile adjoint -> finalize ; direct walk of every photon on overflow
+void launch_backward_tiles(const float* grad, const PhotonArgs& P, const ImageArgs& I, const Geom& g,
+                           const Buffers& B, const GradArgs& G, hipStream_t s) {
+    const unsigned nsegs = (unsigned)g.max_segments;
+    const size_t tile_lds = (size_t)g.T * g.T * 4;
+    hipLaunchKernelGGL(bin_tile_bwd_kernel, dim3(nsegs), dim3(kTileThreads), tile_lds, s, g, I, (const int4*)B.segs,
+                       (const uint4*)B.recs, (const Ctrl*)B.ctrl, grad, B.partials);
+    int64_t blocks = (P.N + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(bin_finalize_kernel, dim3((unsigned)blocks), dim3(256), 0, s, grad, P, I,
+                       (const Ctrl*)B.ctrl, (const uint8_t*)B.info, (const uint32_t*)B.slots,
+                       (const float4*)B.partials, G);
+    hipLaunchKernelGGL(guarded_bwd_direct_kernel, dim3((unsigned)blocks), dim3(256), 0, s, grad, P, I,
+                       (const Ctrl*)B.ctrl, G);
+}
+
+// Photon-order forward for problems the binned path cannot take.
+int forward_unbinned(const PhotonArgs& P, const ImageArgs& I, bool det, float* out, void* ws, size_t ws_bytes,
+                     hipStream_t s) {
+    const int64_t numel = (int64_t)I.C * I.H * I.W;
+    if (!det) {
+        if (launch_fill_zero(out, numel, s) != hipSuccess) return PDS_ERR_HIP;
+        if (!I.ell_trivial) return launch_forward_plain(P, I, out, s) == hipSuccess ? PDS_OK : PDS_ERR_HIP;
+        return launch_forward_flat(P, I, out, s) == hipSuccess ? PDS_OK : PDS_ERR_HIP;
+    }
+    // deterministic: int64 fixed point over the whole image
+    if (ws == nullptr || ws_bytes < align256(sizeof(Ctrl)) + sizeof(uint64_t) * (size_t)numel) {
         g_err = "workspace too small";
         return PDS_ERR_WORKSPACE;
     }
     unsigned char* base = (unsigned char*)ws;
-    uint32_t* hist = (uint32_t*)(base + L.hist);
-    uint32_t* totals = (uint32_t*)(base + L.totals);
-    uint32_t* binoff = (uint32_t*)(base + L.binoff);
-    int4* segs = (int4*)(base + L.segs);
-    Ctrl* ctrl = (Ctrl*)(base + L.ctrl);
-    uint4* recs = (uint4*)(base + L.recs);
-    unsigned long long* acc64 = det ? (unsigned long long*)(base + L.acc64) : nullptr;
-    const unsigned colblocks = (unsigned)((g.nbins + 63) / 64);
-
+    Ctrl* ctrl = (Ctrl*)base;
+    unsigned long long* acc = (unsigned long long*)(base + align256(sizeof(Ctrl)));
     hipLaunchKernelGGL(ctrl_init_kernel, dim3(1), dim3(1), 0, s, ctrl);
-    const size_t hist_lds = sizeof(uint32_t) * (size_t)g.nbins;
-    hipLaunchKernelGGL(bin_count_kernel, dim3(g.G), dim3(kBinThreads), hist_lds, s, P, I, g, hist, ctrl);
-    hipLaunchKernelGGL(bin_columns_kernel<false>, dim3(colblocks), dim3(kColThreads), 0, s, g, hist, totals,
-                       (const uint32_t*)binoff, (const Ctrl*)ctrl);
-    hipLaunchKernelGGL(bin_plan_kernel, dim3(1), dim3(1024), 0, s, g, P.N, (const uint32_t*)totals, binoff, segs,
-                       ctrl);
-    hipLaunchKernelGGL(bin_columns_kernel<true>, dim3(colblocks), dim3(kColThreads), 0, s, g, hist, totals,
-                       (const uint32_t*)binoff, (const Ctrl*)ctrl);
-    hipLaunchKernelGGL(bin_scatter_kernel, dim3(g.G), dim3(kBinThreads), hist_lds, s, P, I, g, (const uint32_t*)hist, recs,
-                       (const Ctrl*)ctrl);
-    const unsigned nsegs = (unsigned)g.max_segments;
-    const size_t tile_lds = (size_t)g.T * g.T * (det ? 8 : 4);
-    if (det) {
-        hipLaunchKernelGGL(bin_zero_split_kernel<unsigned long long>, dim3(nsegs), dim3(kTileThreads), 0, s, g, I,
-                           (const int4*)segs, (const Ctrl*)ctrl, acc64);
-        hipLaunchKernelGGL(bin_tile_kernel<true>, dim3(nsegs), dim3(kTileThreads), tile_lds, s, g, I,
-                           (const int4*)segs, (const uint4*)recs, (const Ctrl*)ctrl, out, acc64);
-        hipLaunchKernelGGL(bin_convert_split_kernel, dim3(nsegs), dim3(kTileThreads), 0, s, g, I, (const int4*)segs,
-                           (const Ctrl*)ctrl, (const unsigned long long*)acc64, out);
-    } else {
-        hipLaunchKernelGGL(bin_zero_split_kernel<float>, dim3(nsegs), dim3(kTileThreads), 0, s, g, I,
-                           (const int4*)segs, (const Ctrl*)ctrl, out);
-        hipLaunchKernelGGL(bin_tile_kernel<false>, dim3(nsegs), dim3(kTileThreads), tile_lds, s, g, I,
-                           (const int4*)segs, (const uint4*)recs, (const Ctrl*)ctrl, out,
-                           (unsigned long long*)nullptr);
+    hipLaunchKernelGGL(emax_kernel, dim3(1024), dim3(256), 0, s, P, I, ctrl);
+    hipLaunchKernelGGL(fixed_point_kernel, dim3(1), dim3(1), 0, s, ctrl, P.N);
+    hipLaunchKernelGGL(guarded_fill64_kernel, dim3(1024), dim3(256), 0, s, (const Ctrl*)ctrl, acc, numel);
+    if (I.ell_trivial)
+        hipLaunchKernelGGL(guarded_flat_kernel<true>, dim3(1024), dim3(256), 0, s, P, I, (const Ctrl*)ctrl, out, acc);
+    else if (P.N > 0)
+        hipLaunchKernelGGL(fwd_plain_fixed_kernel, dim3((unsigned)((P.N + 255) / 256)), dim3(256), 0, s, P, I,
+                           (const Ctrl*)ctrl, acc);
+    hipLaunchKernelGGL(guarded_convert_kernel, dim3(1024), dim3(256), 0, s, (const Ctrl*)ctrl,
+                       (const unsigned long long*)acc, out, numel);
+    return hipGetLastError() == hipSuccess ? PDS_OK : PDS_ERR_HIP;
+}
+
+}  // namespace
+
+size_t binned_workspace_bytes(int64_t N, int32_t C, int32_t H, int32_t W, int32_t cap, int32_t mode,
+                              const pds_tuning* t, bool ell_trivial, bool keep) {
+    if (mode == PDS_MODE_PLAIN) return 0;
+    const bool det = mode == PDS_MODE_DETERMINISTIC;
+    Geom g;
+    if (!ell_trivial || !make_geom(N, C, H, W, cap, mode, t, g)) {
+        // photon-order paths: only the deterministic forward needs the int64 image and a control block
+        if (keep || !det) return 0;
+        return align256(sizeof(Ctrl)) + sizeof(uint64_t) * (size_t)C * H * W;
     }
-    // capacity overflow: the photon-order path, enabled by the device flag
-    const unsigned fb = 1024;
-    if (det) {
-        hipLaunchKernelGGL(guarded_fill64_kernel, dim3(fb), dim3(256), 0, s, (const Ctrl*)ctrl, acc64, numel);
-        hipLaunchKernelGGL(guarded_flat_kernel<true>, dim3(fb), dim3(256), 0, s, P, I, (const Ctrl*)ctrl, out, acc64);
-        hipLaunchKernelGGL(guarded_convert_kernel, dim3(fb), dim3(256), 0, s, (const Ctrl*)ctrl,
-                           (const unsigned long long*)acc64, out, numel);
-    } else {
-        hipLaunchKernelGGL(guarded_fill_kernel, dim3(fb), dim3(256), 0, s, (const Ctrl*)ctrl, out, numel);
-        hipLaunchKernelGGL(guarded_flat_kernel<false>, dim3(fb), dim3(256), 0, s, P, I, (const Ctrl*)ctrl, out,
-                           (unsigned long long*)nullptr);
+    return make_layout(g, N, C, H, W, keep).end;
+}
+
+int binned_forward(const PhotonArgs& P, const ImageArgs& I, int32_t mode, const pds_tuning* t, float* out, void* ws,
+                   size_t ws_bytes, bool keep, hipStream_t s) {
+    const bool det = mode == PDS_MODE_DETERMINISTIC;
+    Geom g;
+    if (!binnable(I, P.N, mode, t, g)) {
+        if (keep) {
+            g_err = "this problem cannot be binned (kept bins unavailable)";
+            return PDS_ERR_ARGUMENT;
+        }
+        return forward_unbinned(P, I, det, out, ws, ws_bytes, s);
     }
-    if (hipGetLastError() != hipSuccess) return PDS_ERR_HIP;
-    return PDS_OK;
+    const Layout L = make_layout(g, P.N, I.C, I.H, I.W, keep);
+    if (ws == nullptr || ws_bytes < L.end) {
+        g_err = "workspace too small";
+        return PDS_ERR_WORKSPACE;
+    }
+    const Buffers B = buffers(ws, L);
+    launch_build(P, I, g, B, keep, s);
+    launch_forward_tiles(P, I, g, B, out, s);
+    return hipGetLastError() == hipSuccess ? PDS_OK : PDS_ERR_HIP;
+}
+
+int binned_backward(const float* grad, const PhotonArgs& P, const ImageArgs& I, int32_t mode, const pds_tuning* t,
+                    const GradArgs& G, void* ws, size_t ws_bytes, bool reuse, hipStream_t s) {
+    Geom g;
+    if (!binnable(I, P.N, mode, t, g)) {
+        g_err = "this problem cannot be binned";
+        return PDS_ERR_ARGUMENT;
+    }
+    const Layout L = make_layout(g, P.N, I.C, I.H, I.W, /*keep=*/true);
+    if (ws == nullptr || ws_bytes < L.end) {
+        g_err = "workspace too small";
+        return PDS_ERR_WORKSPACE;
+    }
+    const Buffers B = buffers(ws, L);
+    if (!reuse) launch_build(P, I, g, B, /*keep=*/true, s);
+    launch_backward_tiles(grad, P, I, g, B, G, s);
+    return hipGetLastError() == hipSuccess ? PDS_OK : PDS_ERR_HIP;
+}
+
+bool binned_possible(const ImageArgs& I, int64_t N, int32_t mode, const pds_tuning* t) {
+    Geom g;
+    return binnable(I, N, mode, t, g);
 }
 
 }  // namespace pds

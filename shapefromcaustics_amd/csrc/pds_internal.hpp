@@ -278,6 +278,47 @@ __device__ __forceinline__ PlainSetup plain_setup(const PhotonArgs& P, const Ima
     return s;
 }
 
+// The seven gradient sums of photon i by a serial walk of its window (the
+// reference's schedule, photon_differentials.cu:172-208).  With clip_box the
+// walk of a photon with finite inputs is limited to its footprint box; the
+// result is the same.
+__device__ __forceinline__ void photon_grad_direct(const float* __restrict__ grad, const PhotonArgs& P,
+                                                   const ImageArgs& I, int64_t i, bool clip_box, float (&a)[7]) {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) a[k] = 0.f;
+    PlainSetup s = plain_setup(P, I, i);
+    if (!s.ok) return;
+    const Frame& f = s.f;
+    if (clip_box) {
+        const int64_t N = P.N;
+        const bool finite = isfinite(f.fx) && isfinite(f.fy) && isfinite(s.e) && isfinite(P.M[i]) &&
+                            isfinite(P.M[N + i]) && isfinite(P.M[3 * N + i]) && isfinite(P.M[4 * N + i]);
+        if (finite) {
+            int32_t x0 = (int32_t)s.x0, x1 = (int32_t)s.x1, y0 = (int32_t)s.y0, y1 = (int32_t)s.y1;
+            clip_to_ellipse(f, x0, x1, y0, y1);
+            s.x0 = x0;
+            s.x1 = x1;
+            s.y0 = y0;
+            s.y1 = y1;
+        }
+    }
+    const float* plane = grad + (int64_t)s.c * I.H * I.W;
+    for (int64_t py = s.y0; py <= s.y1; ++py) {
+        const float dy = (float)((int32_t)py - f.iy) - f.fy;
+        for (int64_t px = s.x0; px <= s.x1; ++px) {
+            if (!I.ell_trivial && !window_ellipse_ok((int32_t)(px - s.win.pxc), (int32_t)(py - s.win.pyc), I.W, I.H))
+                continue;
+            const float dx = (float)((int32_t)px - f.ix) - f.fx;
+            const float u = f.a00 * dx + f.a01 * dy;
+            const float v = f.a10 * dx + f.a11 * dy;
+            const float q = u * u + v * v;
+            const float g = plane[py * I.W + px];
+            const float sc = dk_over_r(q, s.e * g);
+            accumulate_terms(a, q, g, sc, u, v, dx, dy, f);
+        }
+    }
+}
+
 // launchers (pds_kernels.hip)
 hipError_t launch_fill_zero(float* p, int64_t n, hipStream_t s);
 hipError_t launch_forward_plain(const PhotonArgs& P, const ImageArgs& I, float* out, hipStream_t s);

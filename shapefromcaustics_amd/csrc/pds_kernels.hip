@@ -80,27 +80,8 @@ __global__ __launch_bounds__(256) void bwd_plain_kernel(const float* __restrict_
                                                         GradArgs G) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.N) return;
-    const PlainSetup s = plain_setup(P, I, i);
-    float a[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    if (s.ok) {
-        const float* plane = grad + (int64_t)s.c * I.H * I.W;
-        const Frame& f = s.f;
-        for (int64_t py = s.y0; py <= s.y1; ++py) {
-            const float dy = (float)((int32_t)py - f.iy) - f.fy;
-            for (int64_t px = s.x0; px <= s.x1; ++px) {
-                if (!I.ell_trivial &&
-                    !window_ellipse_ok((int32_t)(px - s.win.pxc), (int32_t)(py - s.win.pyc), I.W, I.H))
-                    continue;
-                const float dx = (float)((int32_t)px - f.ix) - f.fx;
-                const float u = f.a00 * dx + f.a01 * dy;
-                const float v = f.a10 * dx + f.a11 * dy;
-                const float q = u * u + v * v;
-                const float g = plane[py * I.W + px];
-                const float sc = dk_over_r(q, s.e * g);
-                accumulate_terms(a, q, g, sc, u, v, dx, dy, f);
-            }
-        }
-    }
+    float a[7];
+    photon_grad_direct(grad, P, I, i, /*clip_box=*/false, a);
     store_grads(G, I, P.N, i, a);
 }
 

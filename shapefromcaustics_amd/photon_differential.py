@@ -16,7 +16,7 @@ import typing as tp
 
 import torch
 
-from .PhotonDifferentialSplatting import pds_backward, pds_forward
+from .PhotonDifferentialSplatting import _backward, _forward
 
 
 class PDS(torch.autograd.Function):
@@ -26,12 +26,19 @@ class PDS(torch.autograd.Function):
         ctx.save_for_backward(Ep, xp, Mp, cp, radius)
         ctx.max_pixel_radius = max_pixel_radius
         ctx.mode = mode
-        return pds_forward(Ep, xp, Mp, cp, radius, output_size, max_pixel_radius, mode=mode)[0]
+        # when a gradient will be needed, the forward leaves its tile binning in
+        # a buffer that the backward reuses instead of binning again
+        keep = any(ctx.needs_input_grad[:3])
+        out, bins = _forward(Ep, xp, Mp, cp, radius, output_size, max_pixel_radius, mode, None, keep_bins=keep)
+        ctx.bins = bins
+        return out
 
     @staticmethod
     def backward(ctx, grad_pds: torch.Tensor):
         Ep, xp, Mp, cp, radius = ctx.saved_tensors
-        grad_Ep, grad_xp, grad_Mp = pds_backward(grad_pds, Ep, xp, Mp, cp, radius, ctx.max_pixel_radius, mode=ctx.mode)
+        grad_Ep, grad_xp, grad_Mp = _backward(grad_pds, Ep, xp, Mp, cp, radius, ctx.max_pixel_radius, ctx.mode, None,
+                                              bins=ctx.bins)
+        ctx.bins = None
         return grad_Ep, grad_xp, grad_Mp, None, None, None, None, None
 
 

@@ -177,7 +177,8 @@ def test_adjoint_matches_central_fd(workload):
     g = WL.upstream_grad(p.size, seed=5)
     rng = np.random.default_rng(1)
     idx = rng.choice(p.n, 64, replace=False)
-    gE, gx, gM, aE, ax, aM = O.backward_f64(g, *p.arrays(), p.cap, idx=idx)
+    ref = O.backward_f64(g, *p.arrays(), p.cap, idx=idx)
+    gx, gM, ax, aM = ref.gx, ref.gM, ref.ax, ref.aM
     W = p.size[2]
     for k, i in enumerate(idx):
         dx, dm, ex, em = O.central_fd(g, p.arrays(), i, p.cap, h_x=1e-3 * 2.0 / W, with_error=True)
@@ -233,7 +234,18 @@ def test_cpu_baseline_matches_oracle():
     got = O.cpu_forward_f32(*p.arrays(), p.size, p.cap)
     m = ref > 1e-3 * ref.max()
     assert (np.abs(got - ref)[m] / ref[m]).max() < 1e-4
-    gE, gx, gM, aE, ax, aM = O.backward_f64(g, *p.arrays(), p.cap)
-    cE, cx, cM = O.cpu_backward_f32(g, *p.arrays(), p.cap)
-    for got_, ref_, mag in ((cE, gE, aE), (cx, gx, ax), (cM, gM, aM)):
-        assert (np.abs(got_ - ref_) <= 1e-3 * (mag + 1e-30)).all()
+    ref = O.backward_f64(g, *p.arrays(), p.cap)
+    ok, worst = O.grads_within_tolerance(O.cpu_backward_f32(g, *p.arrays(), p.cap), ref)
+    assert ok, worst
+
+
+def test_conditioning_bound_is_negligible_for_resolved_footprints():
+    # the FP32 conditioning bound only matters where a footprint barely holds a
+    # pixel: its median is ~3e-6 of the term scale and its 99th percentile
+    # under 1e-4, so for almost every photon the 1e-3 criterion is what the
+    # comparison tests
+    for p in (WL.plain(1 << 14, (1, 256, 256), 8, seed=3), WL.caustic(1 << 14, (3, 256, 256), 12, seed=3)):
+        ref = O.backward_f64(WL.upstream_grad(p.size, seed=3), *p.arrays(), p.cap)
+        for c, a in ((ref.cE, ref.aE), (ref.cx, ref.ax), (ref.cM[:, :2], ref.aM[:, :2])):
+            ratio = c / (a + 1e-300)
+            assert np.median(ratio) < 1e-5 and np.quantile(ratio, 0.99) < 1e-4
