@@ -88,6 +88,8 @@ def main():
     ap.add_argument("--mode", default="auto")
     ap.add_argument("--no-cpu", action="store_true", help="skip the host-CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain-atomic comparator")
+    ap.add_argument("--stateless", action="store_true",
+                    help="time pds_forward + pds_backward (the backward bins again) instead of the PDS pairing")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (photon sharding plus one image "
@@ -98,7 +100,7 @@ def main():
     import torch
 
     from shapefromcaustics_amd import workloads as WL
-    from shapefromcaustics_amd.PhotonDifferentialSplatting import pds_backward, pds_forward
+    from shapefromcaustics_amd.PhotonDifferentialSplatting import _backward, _forward
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda:0")
@@ -113,12 +115,15 @@ def main():
     torch.cuda.synchronize()
 
     def step(mode, ev=None):
+        # the calls PDS.apply / PDS.backward make: the forward keeps its tile
+        # binning for the backward (the stateless module API bins twice)
+        keep = not args.stateless
         if ev is not None:
             ev[0].record()
-        img = pds_forward(E, xp, M, cp, radius, p.size, p.cap, mode=mode)[0]
+        img, bins = _forward(E, xp, M, cp, radius, p.size, p.cap, mode, None, keep_bins=keep)
         if ev is not None:
             ev[1].record()
-        g = pds_backward(grad, E, xp, M, cp, radius, p.cap, mode=mode)
+        g = _backward(grad, E, xp, M, cp, radius, p.cap, mode, None, bins=bins)
         if ev is not None:
             ev[2].record()
         return img, g

@@ -65,7 +65,9 @@ typedef struct pds_tuning {
     int32_t block_photons; /* photons per binning workgroup */
     int32_t capacity_pct;  /* records reserved per 100 photons (default 200); more
                               records than that run the photon-order path */
-    int32_t reserved[4];
+    int32_t unit_fwd;      /* pixels per work unit of the forward tile walk: 2, 4 or 8 */
+    int32_t unit_bwd;      /* pixels per work unit of the backward tile walk: 2, 4 or 8 */
+    int32_t reserved[2];
 } pds_tuning;
 
 /* Bytes of device workspace pds_forward_f32 needs for this problem in

@@ -42,7 +42,8 @@ constexpr int kMaxBins = 16384;
 constexpr int kMaxBinBlocks = 1024;
 constexpr int kColThreads = 1024;
 constexpr int kTileThreads = 256;
-constexpr int kTileChunk = 2;
+constexpr int kTileFwdUnit = 4;
+constexpr int kTileBwdUnit = 8;
 constexpr int kDefaultTile = 32;
 constexpr int kDefaultSeg = 2048;
 constexpr double kDefaultCapacity = 2.0;  // records per photon reserved in the workspace
@@ -61,6 +62,7 @@ struct Ctrl {
 
 struct Geom {
     int32_t T, tshift, tiles_x, tiles_y, nbins, G, seg;
+    int32_t ufwd, ubwd;  // pixels per work unit of the tile kernels
     int64_t per_block, cap_records, max_segments;
     bool det;
 };
@@ -107,6 +109,10 @@ bool make_geom(int64_t N, int32_t C, int32_t H, int32_t W, int32_t cap, int32_t 
     if (g.cap_records >= ((int64_t)1 << 32)) return false;  // 32-bit record offsets
     g.max_segments = g.nbins + g.cap_records / g.seg + 1;
     g.det = mode == PDS_MODE_DETERMINISTIC;
+    g.ufwd = (t && t->unit_fwd > 0) ? t->unit_fwd : kTileFwdUnit;
+    g.ubwd = (t && t->unit_bwd > 0) ? t->unit_bwd : kTileBwdUnit;
+    if (g.ufwd != 2 && g.ufwd != 4 && g.ufwd != 8) return false;
+    if (g.ubwd != 2 && g.ubwd != 4 && g.ubwd != 8) return false;
     return true;
 }
 
@@ -444,6 +450,7 @@ __device__ __forceinline__ float from_fixed(unsigned long long v, int e) {
 // Decode record j for the tile with origin (X0, Y0) and last pixel (X1, Y1)
 // into a flattened-walk record: box = window n tile n ellipse bounding box,
 // LDS tile index of its origin in r.base.
+template <int U>
 __device__ __forceinline__ void decode_record(const uint4* __restrict__ recs, size_t j, int X0, int Y0, int X1, int Y1,
                                               int half, int T, FlatRec& r) {
     const uint4 q0 = recs[2 * j], q1 = recs[2 * j + 1];
@@ -464,19 +471,10 @@ __device__ __forceinline__ void decode_record(const uint4* __restrict__ recs, si
     int x0 = max(max(pxc - rx, X0), 0), x1 = min(pxc + rx, X1);
     int y0 = max(max(pyc - ry, Y0), 0), y1 = min(pyc + ry, Y1);
     clip_to_ellipse(r.f, x0, x1, y0, y1);
-    if (x0 <= x1 && y0 <= y1) {
-        const int bw = x1 - x0 + 1, bh = y1 - y0 + 1;
-        r.cpr = (bw + kTileChunk - 1) / kTileChunk;
-        r.inv_cpr = 1.0f / (float)r.cpr;
-        r.nitems = bh * r.cpr;
-        r.kx0 = x0 - ix;
-        r.ky0 = y0 - iy;
-        r.bw = bw;
-        r.base = (y0 - Y0) * T + (x0 - X0);  // index into the LDS tile
-    }
+    if (x0 <= x1 && y0 <= y1) set_box<U>(r, x0, y0, x1 - x0 + 1, y1 - y0 + 1, (y0 - Y0) * T + (x0 - X0));
 }
 
-template <bool DET>
+template <bool DET, int U>
 __global__ __launch_bounds__(kTileThreads) void bin_tile_kernel(Geom g, ImageArgs I, const int4* __restrict__ segs,
                                                                 const uint4* __restrict__ recs, const Ctrl* ctrl,
                                                                 float* __restrict__ out,
@@ -498,39 +496,31 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_kernel(Geom g, ImageArg
     const int X1 = min(X0 + T, I.W) - 1, Y1 = min(Y0 + T, I.H) - 1;
     const int half = T >> 1;
     __syncthreads();
-    ImageArgs TI = I;
-    TI.W = T;  // walk_items computes base + row * W + col: LDS tile row pitch
 
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     WaveRecs& R = wr[w];
     R.mark[lane] = -1;
     for (int b0 = w * 64; b0 < d.z; b0 += kTileThreads) {
         FlatRec r;
-        r.nitems = 0;
-        r.kx0 = r.ky0 = r.bw = r.base = 0;
-        r.cpr = 1;
-        r.inv_cpr = 1.0f;
-        r.e = 0.f;
-        r.f.fx = r.f.fy = r.f.a00 = r.f.a01 = r.f.a10 = r.f.a11 = 0.f;
-        if (b0 + lane < d.z) decode_record(recs, (size_t)d.y + b0 + lane, X0, Y0, X1, Y1, half, T, r);
+        clear_rec(r);
+        if (b0 + lane < d.z) decode_record<U>(recs, (size_t)d.y + b0 + lane, X0, Y0, X1, Y1, half, T, r);
         const int64_t total = publish_recs(R, r, lane);
-        walk_items<kTileChunk>(R, r, TI, lane, total, [&](const FlatItem& it) {
+        walk_units<U>(R, r, lane, total, [&](const FlatItem& it) {
+            const UnitPixels<U> px(it, T);
 #pragma unroll
-            for (int k = 0; k < kTileChunk; ++k) {
+            for (int k = 0; k < U; ++k) {
                 if (k < it.n) {
-                    const float dx = (float)(it.kx + k) - it.f.fx;
-                    const float u = it.f.a00 * dx + it.f.a01 * it.dy;
-                    const float v = it.f.a10 * dx + it.f.a11 * it.dy;
-                    const float val = silverman(u * u + v * v) * it.e;
+                    float u, v;
+                    const float val = silverman(eval_q(it.f, px.dx[k], px.dy[k], u, v)) * it.e;
                     if (val > 0.0f) {
                         if constexpr (DET) {
                             const unsigned long long fxv = to_fixed(val, fe);
                             if (fxv >> 62)
-                                atomicOr(&tile[it.pix + k], fxv);
+                                atomicOr(&tile[px.pix[k]], fxv);
                             else
-                                atomicAdd(&tile[it.pix + k], fxv);
+                                atomicAdd(&tile[px.pix[k]], fxv);
                         } else {
-                            atomicAdd(&tile[it.pix + k], val);
+                            atomicAdd(&tile[px.pix[k]], val);
                         }
                     }
                 }
@@ -589,6 +579,7 @@ __global__ __launch_bounds__(kTileThreads) void bin_convert_split_kernel(Geom g,
 // record's seven sums over its pixels in the tile are reduced in LDS and
 // stored, in the reference's units, at the record's slot (coalesced).
 // ---------------------------------------------------------------------------
+template <int U>
 __global__ __launch_bounds__(kTileThreads) void bin_tile_bwd_kernel(Geom g, ImageArgs I, const int4* __restrict__ segs,
                                                                     const uint4* __restrict__ recs, const Ctrl* ctrl,
                                                                     const float* __restrict__ grad,
@@ -617,31 +608,23 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_bwd_kernel(Geom g, Imag
     for (int k = 0; k < 7; ++k) acc[w][k][lane] = 0.f;
     __syncthreads();
     const float hw = 0.5f * I.fw, hh = 0.5f * I.fh, iw = 2.0f / I.fw, ih = 2.0f / I.fh;
-    ImageArgs TI = I;
-    TI.W = T;  // walk_items: LDS tile row pitch
     for (int b0 = w * 64; b0 < d.z; b0 += kTileThreads) {
         FlatRec r;
-        r.nitems = 0;
-        r.kx0 = r.ky0 = r.bw = r.base = 0;
-        r.cpr = 1;
-        r.inv_cpr = 1.0f;
-        r.e = 0.f;
-        r.f.fx = r.f.fy = r.f.a00 = r.f.a01 = r.f.a10 = r.f.a11 = 0.f;
+        clear_rec(r);
         const size_t j = (size_t)d.y + b0 + lane;
-        if (b0 + lane < d.z) decode_record(recs, j, X0, Y0, X1, Y1, half, T, r);
+        if (b0 + lane < d.z) decode_record<U>(recs, j, X0, Y0, X1, Y1, half, T, r);
         const int64_t total = publish_recs(R, r, lane);
-        walk_items<kTileChunk>(R, r, TI, lane, total, [&](const FlatItem& it) {
+        walk_units<U>(R, r, lane, total, [&](const FlatItem& it) {
+            const UnitPixels<U> px(it, T);
             float a[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int k = 0; k < kTileChunk; ++k) {
+            for (int k = 0; k < U; ++k) {
                 if (k < it.n) {
-                    const float dx = (float)(it.kx + k) - it.f.fx;
-                    const float u = it.f.a00 * dx + it.f.a01 * it.dy;
-                    const float v = it.f.a10 * dx + it.f.a11 * it.dy;
-                    const float q = u * u + v * v;
-                    const float gg = gt[it.pix + k];
+                    float u, v;
+                    const float q = eval_q(it.f, px.dx[k], px.dy[k], u, v);
+                    const float gg = gt[px.pix[k]];
                     const float sc = dk_over_r(q, it.e * gg);
-                    accumulate_terms(a, q, gg, sc, u, v, dx, it.dy, it.f);
+                    accumulate_terms(a, q, gg, sc, u, v, px.dx[k], px.dy[k], it.f);
                 }
             }
 #pragma unroll
@@ -755,28 +738,27 @@ __global__ __launch_bounds__(256) void guarded_flat_kernel(PhotonArgs P, ImageAr
     for (int64_t b = (int64_t)blockIdx.x * 4 + w; b < nbatch; b += (int64_t)gridDim.x * 4) {
         const int64_t i = (b << 6) + lane;
         FlatRec r;
-        setup_flat_rec<2>(P, I, i, /*forward=*/true, r);
+        setup_flat_rec<kFwdUnit>(P, I, i, /*forward=*/true, r);
         const int64_t total = publish_recs(R, r, lane);
-        walk_items<2>(R, r, I, lane, total, [&](const FlatItem& it) {
+        walk_units<kFwdUnit>(R, r, lane, total, [&](const FlatItem& it) {
+            if constexpr (DET) {
+                const UnitPixels<kFwdUnit> px(it, I.W);
 #pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                if (k < it.n) {
-                    const float dx = (float)(it.kx + k) - it.f.fx;
-                    const float u = it.f.a00 * dx + it.f.a01 * it.dy;
-                    const float v = it.f.a10 * dx + it.f.a11 * it.dy;
-                    const float val = silverman(u * u + v * v) * it.e;
-                    if (val > 0.0f) {
-                        if constexpr (DET) {
+                for (int k = 0; k < kFwdUnit; ++k) {
+                    if (k < it.n) {
+                        float u, v;
+                        const float val = silverman(eval_q(it.f, px.dx[k], px.dy[k], u, v)) * it.e;
+                        if (val > 0.0f) {
                             const unsigned long long fxv = to_fixed(val, fe);
                             if (fxv >> 62)
-                                atomicOr(&acc64[it.pix + k], fxv);
+                                atomicOr(&acc64[px.pix[k]], fxv);
                             else
-                                atomicAdd(&acc64[it.pix + k], fxv);
-                        } else {
-                            unsafeAtomicAdd(&out[it.pix + k], val);
+                                atomicAdd(&acc64[px.pix[k]], fxv);
                         }
                     }
                 }
+            } else {
+                flat_forward_unit<kFwdUnit>(it, I.W, out);
             }
         });
     }
@@ -824,9 +806,8 @@ __global__ __launch_bounds__(256) void fwd_plain_fixed_kernel(PhotonArgs P, Imag
             if (!I.ell_trivial && !window_ellipse_ok((int32_t)(px - s.win.pxc), (int32_t)(py - s.win.pyc), I.W, I.H))
                 continue;
             const float dx = (float)((int32_t)px - f.ix) - f.fx;
-            const float u = f.a00 * dx + f.a01 * dy;
-            const float v = f.a10 * dx + f.a11 * dy;
-            const float val = silverman(u * u + v * v) * s.e;
+            float u, v;
+            const float val = silverman(eval_q(f, dx, dy, u, v)) * s.e;
             if (val > 0.0f) {
                 const unsigned long long q = to_fixed(val, fe);
                 if (q >> 62)
@@ -906,6 +887,21 @@ void launch_build(const PhotonArgs& P, const ImageArgs& I, const Geom& g, const 
                            (const Ctrl*)B.ctrl);
 }
 
+template <bool DET>
+void launch_tile_fwd(const Geom& g, const ImageArgs& I, const Buffers& B, float* out, size_t lds, hipStream_t s) {
+    const dim3 grid((unsigned)g.max_segments), block(kTileThreads);
+#define PDS_TILE_FWD(U)                                                                                           \
+    hipLaunchKernelGGL((bin_tile_kernel<DET, U>), grid, block, lds, s, g, I, (const int4*)B.segs,                 \
+                       (const uint4*)B.recs, (const Ctrl*)B.ctrl, out, B.acc64)
+    if (g.ufwd == 2)
+        PDS_TILE_FWD(2);
+    else if (g.ufwd == 8)
+        PDS_TILE_FWD(8);
+    else
+        PDS_TILE_FWD(4);
+#undef PDS_TILE_FWD
+}
+
 // zero split tiles -> tiles -> (convert) ; photon-order path on overflow
 void launch_forward_tiles(const PhotonArgs& P, const ImageArgs& I, const Geom& g, const Buffers& B, float* out,
                           hipStream_t s) {
@@ -916,8 +912,7 @@ void launch_forward_tiles(const PhotonArgs& P, const ImageArgs& I, const Geom& g
     if (g.det) {
         hipLaunchKernelGGL(bin_zero_split_kernel<unsigned long long>, dim3(nsegs), dim3(kTileThreads), 0, s, g, I,
                            (const int4*)B.segs, (const Ctrl*)B.ctrl, B.acc64);
-        hipLaunchKernelGGL(bin_tile_kernel<true>, dim3(nsegs), dim3(kTileThreads), tile_lds, s, g, I,
-                           (const int4*)B.segs, (const uint4*)B.recs, (const Ctrl*)B.ctrl, out, B.acc64);
+        launch_tile_fwd<true>(g, I, B, out, tile_lds, s);
         hipLaunchKernelGGL(bin_convert_split_kernel, dim3(nsegs), dim3(kTileThreads), 0, s, g, I,
                            (const int4*)B.segs, (const Ctrl*)B.ctrl, (const unsigned long long*)B.acc64, out);
         hipLaunchKernelGGL(guarded_fill64_kernel, dim3(fb), dim3(256), 0, s, (const Ctrl*)B.ctrl, B.acc64, numel);
@@ -928,9 +923,7 @@ void launch_forward_tiles(const PhotonArgs& P, const ImageArgs& I, const Geom& g
     } else {
         hipLaunchKernelGGL(bin_zero_split_kernel<float>, dim3(nsegs), dim3(kTileThreads), 0, s, g, I,
                            (const int4*)B.segs, (const Ctrl*)B.ctrl, out);
-        hipLaunchKernelGGL(bin_tile_kernel<false>, dim3(nsegs), dim3(kTileThreads), tile_lds, s, g, I,
-                           (const int4*)B.segs, (const uint4*)B.recs, (const Ctrl*)B.ctrl, out,
-                           (unsigned long long*)nullptr);
+        launch_tile_fwd<false>(g, I, B, out, tile_lds, s);
         hipLaunchKernelGGL(guarded_fill_kernel, dim3(fb), dim3(256), 0, s, (const Ctrl*)B.ctrl, out, numel);
         hipLaunchKernelGGL(guarded_flat_kernel<false>, dim3(fb), dim3(256), 0, s, P, I, (const Ctrl*)B.ctrl, out,
                            (unsigned long long*)nullptr);
@@ -942,8 +935,16 @@ void launch_backward_tiles(const float* grad, const PhotonArgs& P, const ImageAr
                            const Buffers& B, const GradArgs& G, hipStream_t s) {
     const unsigned nsegs = (unsigned)g.max_segments;
     const size_t tile_lds = (size_t)g.T * g.T * 4;
-    hipLaunchKernelGGL(bin_tile_bwd_kernel, dim3(nsegs), dim3(kTileThreads), tile_lds, s, g, I, (const int4*)B.segs,
-                       (const uint4*)B.recs, (const Ctrl*)B.ctrl, grad, B.partials);
+#define PDS_TILE_BWD(U)                                                                                           \
+    hipLaunchKernelGGL((bin_tile_bwd_kernel<U>), dim3(nsegs), dim3(kTileThreads), tile_lds, s, g, I,               \
+                       (const int4*)B.segs, (const uint4*)B.recs, (const Ctrl*)B.ctrl, grad, B.partials)
+    if (g.ubwd == 2)
+        PDS_TILE_BWD(2);
+    else if (g.ubwd == 4)
+        PDS_TILE_BWD(4);
+    else
+        PDS_TILE_BWD(8);
+#undef PDS_TILE_BWD
     int64_t blocks = (P.N + 255) / 256;
     if (blocks > 8192) blocks = 8192;
     if (blocks < 1) blocks = 1;

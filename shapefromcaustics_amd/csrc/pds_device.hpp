@@ -127,6 +127,16 @@ __device__ __forceinline__ void clip_to_ellipse(const Frame& f, int32_t& x0, int
     y1 = min(y1, f.iy + (int32_t)floorf(f.fy + hy));
 }
 
+// (u, v) = A (dx, dy) and q = u^2 + v^2 with the fused multiply-adds written
+// out, so that every kernel (and every position of a pixel within a work
+// unit) evaluates a pixel bit-identically; compiler contraction could
+// otherwise differ between unrolled copies.
+__device__ __forceinline__ float eval_q(const Frame& f, float dx, float dy, float& u, float& v) {
+    u = fmaf(f.a00, dx, f.a01 * dy);
+    v = fmaf(f.a10, dx, f.a11 * dy);
+    return fmaf(u, u, v * v);
+}
+
 __device__ __forceinline__ float silverman(float q) {
     const float t = 1.0f - q;
     return q < 1.0f ? kThreeOverPi * t * t : 0.0f;

@@ -10,9 +10,9 @@
 
 namespace pds {
 
-// Pixels per work item of the flattened walk (see walk_items).
-constexpr int kFwdChunk = 2;
-constexpr int kBwdChunk = 2;
+// Pixels per work item ("unit") of the flattened walk (see walk_units).
+constexpr int kFwdUnit = 4;
+constexpr int kBwdUnit = 8;
 
 // Photon inputs in the reference layout (PhotonDifferentialSplattig.cpp:19-22).
 struct PhotonArgs {
@@ -96,22 +96,46 @@ __device__ __forceinline__ int64_t wave_inclusive_sum(int64_t v) {
     return (int64_t)o;
 }
 
-// One photon's footprint box, prepared for the flattened walk.
+// One photon's footprint box, prepared for the flattened walk.  The box's
+// pixels are numbered row-major; a work unit is U consecutive pixels of that
+// order (it may wrap to the next row), so units carry the same work whatever
+// the box shape and a photon needs ceil(area / U) of them.
 struct FlatRec {
     Frame f;
-    float e, inv_cpr;
+    float e, inv_bw;
     int32_t kx0, ky0;          // box origin relative to (ix, iy)
-    int32_t cpr, bw;           // chunks per row, box width
-    int32_t base;              // flat index of the box origin pixel in [C,H,W]
-    int32_t nitems;            // rows * chunks per row (0: nothing to do)
+    int32_t bw, area;          // box width, box area
+    int32_t base;              // flat index of the box origin (image or LDS tile)
+    int32_t nitems;            // units (0: nothing to do)
     int64_t start;             // exclusive prefix of nitems over the wavefront
 };
 
+__device__ __forceinline__ void clear_rec(FlatRec& r) {
+    r.nitems = 0;
+    r.kx0 = r.ky0 = r.base = r.area = 0;
+    r.bw = 1;
+    r.inv_bw = 1.0f;
+    r.e = 0.f;
+    r.f.ix = r.f.iy = 0;
+    r.f.fx = r.f.fy = r.f.a00 = r.f.a01 = r.f.a10 = r.f.a11 = 0.f;
+}
+
+template <int U>
+__device__ __forceinline__ void set_box(FlatRec& r, int32_t x0, int32_t y0, int32_t bw, int32_t bh, int32_t base) {
+    r.bw = bw;
+    r.inv_bw = 1.0f / (float)bw;
+    r.area = bw * bh;
+    r.nitems = (r.area + U - 1) / U;
+    r.kx0 = x0 - r.f.ix;
+    r.ky0 = y0 - r.f.iy;
+    r.base = base;
+}
+
 // Per-wavefront LDS copy of the 64 records of the current batch.
 struct WaveRecs {
-    float4 A[64];      // fx, fy, e, inv_cpr
+    float4 A[64];      // fx, fy, e, inv_bw
     float4 B[64];      // a00, a01, a10, a11
-    int4 Cq[64];       // kx0, ky0, cpr, bw
+    int4 Cq[64];       // kx0, ky0, bw, area
     int32_t base[64];
     int64_t start[64];
     int32_t mark[64];  // record whose first item lands on this lane (-1: none)
@@ -121,18 +145,12 @@ struct WaveRecs {
 // clipped to the image, then -- for photons whose inputs are all finite -- to
 // the bounding box of the ellipse |M d| < 1 (with a 1e-4 relative safety
 // margin).  Pixels outside that ellipse have K = 0 and contribute exactly
-// nothing, so the result is unchanged.  Non-finite photons keep the whole window so that NaN
-// propagates into their gradients the way it does in the reference.
-template <int CH>
+// nothing, so the result is unchanged.  Non-finite photons keep the whole
+// window so that NaN propagates into their gradients as in the reference.
+template <int U>
 __device__ __forceinline__ void setup_flat_rec(const PhotonArgs& P, const ImageArgs& I, int64_t i, bool forward,
                                                FlatRec& r) {
-    r.nitems = 0;
-    r.kx0 = r.ky0 = r.bw = r.base = 0;
-    r.cpr = 1;
-    r.inv_cpr = 1.0f;
-    r.e = 0.f;
-    r.f.ix = r.f.iy = 0;
-    r.f.fx = r.f.fy = r.f.a00 = r.f.a01 = r.f.a10 = r.f.a11 = 0.f;
+    clear_rec(r);
     if (i >= P.N) return;
     const int64_t c = P.cp[i];
     if (c < 0 || c >= I.C) return;
@@ -142,10 +160,10 @@ __device__ __forceinline__ void setup_flat_rec(const PhotonArgs& P, const ImageA
     if (forward && !(e > 0.0f)) return;  // val = K E > 0 never holds (:106)
     const Window win = photon_window(P.radius[i], cx, cy, I.W, I.H, I.cap);
     if (win.rx < 0 || win.ry < 0) return;
-    int64_t x0 = max((int64_t)win.pxc - win.rx, (int64_t)0);
-    int64_t x1 = min((int64_t)win.pxc + win.rx, (int64_t)I.W - 1);
-    int64_t y0 = max((int64_t)win.pyc - win.ry, (int64_t)0);
-    int64_t y1 = min((int64_t)win.pyc + win.ry, (int64_t)I.H - 1);
+    int32_t x0 = (int32_t)max((int64_t)win.pxc - win.rx, (int64_t)0);
+    int32_t x1 = (int32_t)min((int64_t)win.pxc + win.rx, (int64_t)I.W - 1);
+    int32_t y0 = (int32_t)max((int64_t)win.pyc - win.ry, (int64_t)0);
+    int32_t y1 = (int32_t)min((int64_t)win.pyc + win.ry, (int64_t)I.H - 1);
     if (x0 > x1 || y0 > y1) return;
     const float m00 = P.M[i], m01 = P.M[N + i], m10 = P.M[3 * N + i], m11 = P.M[4 * N + i];
     const float poison = P.M[2 * N + i] * 0.0f + P.M[5 * N + i] * 0.0f;  // NaN iff M[:,2] is not finite
@@ -153,24 +171,12 @@ __device__ __forceinline__ void setup_flat_rec(const PhotonArgs& P, const ImageA
     const bool finite = isfinite(cx) && isfinite(cy) && isfinite(e) && isfinite(m00) && isfinite(m01) &&
                         isfinite(m10) && isfinite(m11) && poison == 0.0f;
     if (finite) {
-        int32_t a0 = (int32_t)x0, a1 = (int32_t)x1, b0 = (int32_t)y0, b1 = (int32_t)y1;
-        clip_to_ellipse(f, a0, a1, b0, b1);
-        if (a0 > a1 || b0 > b1) return;
-        x0 = a0;
-        x1 = a1;
-        y0 = b0;
-        y1 = b1;
+        clip_to_ellipse(f, x0, x1, y0, y1);
+        if (x0 > x1 || y0 > y1) return;
     }
-    const int32_t bw = (int32_t)(x1 - x0 + 1), bh = (int32_t)(y1 - y0 + 1);
     r.f = f;
     r.e = e;
-    r.cpr = (bw + CH - 1) / CH;
-    r.inv_cpr = 1.0f / (float)r.cpr;
-    r.nitems = bh * r.cpr;
-    r.kx0 = (int32_t)x0 - f.ix;
-    r.ky0 = (int32_t)y0 - f.iy;
-    r.bw = bw;
-    r.base = (int32_t)((c * I.H + y0) * I.W + x0);
+    set_box<U>(r, x0, y0, x1 - x0 + 1, y1 - y0 + 1, (int32_t)((c * I.H + y0) * I.W + x0));
 }
 
 // Publish this lane's record to the wavefront's LDS table; returns the total
@@ -179,33 +185,32 @@ __device__ __forceinline__ int64_t publish_recs(WaveRecs& R, FlatRec& r, int lan
     const int64_t incl = wave_inclusive_sum((int64_t)r.nitems);
     r.start = incl - r.nitems;
     const int64_t total = __shfl(incl, 63);
-    R.A[lane] = make_float4(r.f.fx, r.f.fy, r.e, r.inv_cpr);
+    R.A[lane] = make_float4(r.f.fx, r.f.fy, r.e, r.inv_bw);
     R.B[lane] = make_float4(r.f.a00, r.f.a01, r.f.a10, r.f.a11);
-    R.Cq[lane] = make_int4(r.kx0, r.ky0, r.cpr, r.bw);
+    R.Cq[lane] = make_int4(r.kx0, r.ky0, r.bw, r.area);
     R.base[lane] = r.base;
     R.start[lane] = r.start;
     wave_lds_fence();
     return total;
 }
 
-// One work item: up to CH consecutive pixels of one row of one photon's box.
+// One work unit: up to U consecutive pixels (row-major) of one photon's box.
 struct FlatItem {
-    int32_t rec;             // lane (record) the item belongs to
-    int32_t kx;              // x offset of the first pixel from the photon's ix
-    int32_t n;               // pixels in this item (1..CH)
-    int32_t pix;             // flat index of the first pixel in the [C,H,W] image
-    float dy;                // row offset in pixels, (py - iy) - fy
+    int32_t rec;              // lane (record) the unit belongs to
+    int32_t row0, col0;       // first pixel, relative to the box origin
+    int32_t n;                // pixels in this unit (1..U)
+    int32_t kx0, ky0, bw;     // box origin relative to (ix, iy), box width
+    int32_t base;             // flat index of the box origin
     float e;
-    Frame f;                 // fx, a.. valid (ix, iy, fy unused)
+    Frame f;                  // fx, fy, a.. valid (ix, iy unused)
 };
 
-// Walk all work items of the batch, 64 per step.  The lane -> record map of a
+// Walk all work units of the batch, 64 per step.  The lane -> record map of a
 // step is an inclusive max-scan of "record starts here" marks: records are in
-// lane order and their item ranges are contiguous, so the owner of item j is
+// lane order and their unit ranges are contiguous, so the owner of unit j is
 // the last record that started at or before j.
-template <int CH, class F>
-__device__ __forceinline__ void walk_items(WaveRecs& R, const FlatRec& r, const ImageArgs& I, int lane, int64_t total,
-                                           F&& body) {
+template <int U, class F>
+__device__ __forceinline__ void walk_units(WaveRecs& R, const FlatRec& r, int lane, int64_t total, F&& body) {
     int carry = -1;
     for (int64_t base = 0; base < total; base += 64) {
         if (r.nitems > 0 && r.start >= base && r.start < base + 64) R.mark[(int)(r.start - base)] = lane;
@@ -223,16 +228,29 @@ __device__ __forceinline__ void walk_items(WaveRecs& R, const FlatRec& r, const 
             const float4 A = R.A[m];
             const float4 B = R.B[m];
             const int4 Q = R.Cq[m];
-            const int32_t local = (int32_t)(item - R.start[m]);
-            int32_t row = (int32_t)(((float)local + 0.5f) * A.w);
-            if (local >= (1 << 20)) row = local / Q.z;  // the float quotient is exact below 2^20
-            const int32_t chunk = local - row * Q.z;
-            it.kx = Q.x + chunk * CH;
-            it.n = min(CH, Q.w - chunk * CH);
-            it.pix = R.base[m] + row * I.W + chunk * CH;
-            it.dy = (float)(Q.y + row) - A.y;
+            const int32_t p0 = (int32_t)(item - R.start[m]) * U;
+            // row = p0 / bw: the float estimate is within one of the quotient
+            // for boxes of up to 2^18 pixels and is corrected exactly
+            int32_t row = (int32_t)(((float)p0 + 0.5f) * A.w);
+            if (p0 >= (1 << 18)) row = p0 / Q.z;
+            int32_t col = p0 - row * Q.z;
+            if (col < 0) {
+                --row;
+                col += Q.z;
+            } else if (col >= Q.z) {
+                ++row;
+                col -= Q.z;
+            }
+            it.row0 = row;
+            it.col0 = col;
+            it.n = min(U, Q.w - p0);
+            it.kx0 = Q.x;
+            it.ky0 = Q.y;
+            it.bw = Q.z;
+            it.base = R.base[m];
             it.e = A.z;
             it.f.fx = A.x;
+            it.f.fy = A.y;
             it.f.a00 = B.x;
             it.f.a01 = B.y;
             it.f.a10 = B.z;
@@ -242,6 +260,28 @@ __device__ __forceinline__ void walk_items(WaveRecs& R, const FlatRec& r, const 
         wave_lds_fence();
     }
 }
+
+// Pixel k of a unit: offsets (dx, dy) from the photon centre in pixels and the
+// flat index (row pitch ``pitch``).  Units are short, so the row/column of
+// pixel k is stepped from the unit's first pixel.
+template <int U>
+struct UnitPixels {
+    float dx[U], dy[U];
+    int32_t pix[U];
+    __device__ __forceinline__ UnitPixels(const FlatItem& it, int32_t pitch) {
+        int32_t row = it.row0, col = it.col0;
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            dx[k] = (float)(it.kx0 + col) - it.f.fx;
+            dy[k] = (float)(it.ky0 + row) - it.f.fy;
+            pix[k] = it.base + row * pitch + col;
+            if (++col == it.bw) {
+                col = 0;
+                ++row;
+            }
+        }
+    }
+};
 
 // Per-photon setup of the plain (reference-schedule) kernels: the reference
 // window (photon_differentials.cu:74-89) clipped to the image, and the frame.
@@ -309,9 +349,8 @@ __device__ __forceinline__ void photon_grad_direct(const float* __restrict__ gra
             if (!I.ell_trivial && !window_ellipse_ok((int32_t)(px - s.win.pxc), (int32_t)(py - s.win.pyc), I.W, I.H))
                 continue;
             const float dx = (float)((int32_t)px - f.ix) - f.fx;
-            const float u = f.a00 * dx + f.a01 * dy;
-            const float v = f.a10 * dx + f.a11 * dy;
-            const float q = u * u + v * v;
+            float u, v;
+            const float q = eval_q(f, dx, dy, u, v);
             const float g = plane[py * I.W + px];
             const float sc = dk_over_r(q, s.e * g);
             accumulate_terms(a, q, g, sc, u, v, dx, dy, f);
@@ -325,6 +364,20 @@ hipError_t launch_forward_plain(const PhotonArgs& P, const ImageArgs& I, float* 
 hipError_t launch_backward_plain(const float* grad, const PhotonArgs& P, const ImageArgs& I, const GradArgs& G,
                                  hipStream_t s);
 hipError_t launch_forward_flat(const PhotonArgs& P, const ImageArgs& I, float* out, hipStream_t s);
+// The flat forward's body (global float atomics), shared with the binned
+// module's capacity-overflow path.
+template <int U>
+__device__ __forceinline__ void flat_forward_unit(const FlatItem& it, int32_t W, float* __restrict__ out) {
+    const UnitPixels<U> px(it, W);
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+        if (k < it.n) {
+            float u, v;
+            const float val = silverman(eval_q(it.f, px.dx[k], px.dy[k], u, v)) * it.e;
+            if (val > 0.0f) unsafeAtomicAdd(&out[px.pix[k]], val);
+        }
+    }
+}
 hipError_t launch_backward_flat(const float* grad, const PhotonArgs& P, const ImageArgs& I, const GradArgs& G,
                                 hipStream_t s);
 

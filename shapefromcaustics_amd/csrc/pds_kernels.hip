@@ -68,9 +68,8 @@ __global__ __launch_bounds__(256) void fwd_plain_kernel(PhotonArgs P, ImageArgs 
             if (!I.ell_trivial && !window_ellipse_ok((int32_t)(px - s.win.pxc), (int32_t)(py - s.win.pyc), I.W, I.H))
                 continue;
             const float dx = (float)((int32_t)px - f.ix) - f.fx;
-            const float u = f.a00 * dx + f.a01 * dy;
-            const float v = f.a10 * dx + f.a11 * dy;
-            const float val = silverman(u * u + v * v) * s.e;
+            float u, v;
+            const float val = silverman(eval_q(f, dx, dy, u, v)) * s.e;
             if (val > 0.0f) unsafeAtomicAdd(&plane[py * I.W + px], val);
         }
     }
@@ -90,7 +89,7 @@ __global__ __launch_bounds__(256) void bwd_plain_kernel(const float* __restrict_
 // ---------------------------------------------------------------------------
 
 // Forward: global FP32 atomics into the zero-filled image.
-template <int CH>
+template <int U>
 __global__ __launch_bounds__(256) void fwd_flat_kernel(PhotonArgs P, ImageArgs I, float* __restrict__ out) {
     __shared__ WaveRecs recs[4];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -100,25 +99,14 @@ __global__ __launch_bounds__(256) void fwd_flat_kernel(PhotonArgs P, ImageArgs I
     for (int64_t b = (int64_t)blockIdx.x * 4 + w; b < nbatch; b += (int64_t)gridDim.x * 4) {
         const int64_t i = (b << 6) + lane;
         FlatRec r;
-        setup_flat_rec<CH>(P, I, i, /*forward=*/true, r);
-        int64_t total = publish_recs(R, r, lane);
-        walk_items<CH>(R, r, I, lane, total, [&](const FlatItem& it) {
-#pragma unroll
-            for (int k = 0; k < CH; ++k) {
-                if (k < it.n) {
-                    const float dx = (float)(it.kx + k) - it.f.fx;
-                    const float u = it.f.a00 * dx + it.f.a01 * it.dy;
-                    const float v = it.f.a10 * dx + it.f.a11 * it.dy;
-                    const float val = silverman(u * u + v * v) * it.e;
-                    if (val > 0.0f) unsafeAtomicAdd(&out[it.pix + k], val);
-                }
-            }
-        });
+        setup_flat_rec<U>(P, I, i, /*forward=*/true, r);
+        const int64_t total = publish_recs(R, r, lane);
+        walk_units<U>(R, r, lane, total, [&](const FlatItem& it) { flat_forward_unit<U>(it, I.W, out); });
     }
 }
 
 // Backward: gather + per-photon LDS reduction, coalesced stores.
-template <int CH>
+template <int U>
 __global__ __launch_bounds__(256) void bwd_flat_kernel(const float* __restrict__ grad, PhotonArgs P, ImageArgs I,
                                                        GradArgs G) {
     __shared__ WaveRecs recs[4];
@@ -132,23 +120,21 @@ __global__ __launch_bounds__(256) void bwd_flat_kernel(const float* __restrict__
     for (int64_t b = (int64_t)blockIdx.x * 4 + w; b < nbatch; b += (int64_t)gridDim.x * 4) {
         const int64_t i = (b << 6) + lane;
         FlatRec r;
-        setup_flat_rec<CH>(P, I, i, /*forward=*/false, r);
-        int64_t total = publish_recs(R, r, lane);
-        walk_items<CH>(R, r, I, lane, total, [&](const FlatItem& it) {
+        setup_flat_rec<U>(P, I, i, /*forward=*/false, r);
+        const int64_t total = publish_recs(R, r, lane);
+        walk_units<U>(R, r, lane, total, [&](const FlatItem& it) {
+            const UnitPixels<U> px(it, I.W);
+            float g[U];  // every gather of the unit is issued before any is used
+#pragma unroll
+            for (int k = 0; k < U; ++k) g[k] = grad[k < it.n ? px.pix[k] : px.pix[0]];
             float a[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-            // issue every gather of the item before using any of them
-            float g[CH];
 #pragma unroll
-            for (int k = 0; k < CH; ++k) g[k] = grad[it.pix + (k < it.n ? k : 0)];
-#pragma unroll
-            for (int k = 0; k < CH; ++k) {
+            for (int k = 0; k < U; ++k) {
                 if (k < it.n) {
-                    const float dx = (float)(it.kx + k) - it.f.fx;
-                    const float u = it.f.a00 * dx + it.f.a01 * it.dy;
-                    const float v = it.f.a10 * dx + it.f.a11 * it.dy;
-                    const float q = u * u + v * v;
+                    float u, v;
+                    const float q = eval_q(it.f, px.dx[k], px.dy[k], u, v);
                     const float sc = dk_over_r(q, it.e * g[k]);
-                    accumulate_terms(a, q, g[k], sc, u, v, dx, it.dy, it.f);
+                    accumulate_terms(a, q, g[k], sc, u, v, px.dx[k], px.dy[k], it.f);
                 }
             }
 #pragma unroll
@@ -194,14 +180,14 @@ hipError_t launch_backward_plain(const float* grad, const PhotonArgs& P, const I
 
 hipError_t launch_forward_flat(const PhotonArgs& P, const ImageArgs& I, float* out, hipStream_t s) {
     if (P.N == 0) return hipSuccess;
-    hipLaunchKernelGGL(fwd_flat_kernel<kFwdChunk>, dim3(flat_grid(P.N)), dim3(256), 0, s, P, I, out);
+    hipLaunchKernelGGL(fwd_flat_kernel<kFwdUnit>, dim3(flat_grid(P.N)), dim3(256), 0, s, P, I, out);
     return hipGetLastError();
 }
 
 hipError_t launch_backward_flat(const float* grad, const PhotonArgs& P, const ImageArgs& I, const GradArgs& G,
                                 hipStream_t s) {
     if (P.N == 0) return hipSuccess;
-    hipLaunchKernelGGL(bwd_flat_kernel<kBwdChunk>, dim3(flat_grid(P.N)), dim3(256), 0, s, grad, P, I, G);
+    hipLaunchKernelGGL(bwd_flat_kernel<kBwdUnit>, dim3(flat_grid(P.N)), dim3(256), 0, s, grad, P, I, G);
     return hipGetLastError();
 }
 

@@ -51,7 +51,7 @@ def _run(p, mode, grad=None, tuning=None):
     if grad is None:
         return img
     g = torch.from_numpy(grad).cuda()
-    return img, pds_backward(g, E, xp, M, cp, radius, p.cap, mode=mode)
+    return img, pds_backward(g, E, xp, M, cp, radius, p.cap, mode=mode, tuning=tuning)
 
 
 # ----------------------------------------------------------------------------- small mixed cases
@@ -286,3 +286,13 @@ def test_backward_repeatable():
     for r in runs[1:]:
         for a, b in zip(runs[0], r):
             assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("uf,ub", [(2, 2), (4, 4), (8, 8), (2, 8)])
+def test_unit_sizes(uf, ub):
+    p = WL.caustic(1 << 16, (3, 192, 160), 12, seed=uf * 10 + ub)
+    g = WL.upstream_grad(p.size, seed=3)
+    tun = {"unit_fwd": uf, "unit_bwd": ub, "tile": 16}
+    img, grads = _run(p, "auto", g, tuning=tun)
+    assert_image_close(img, O.forward_f64(*p.arrays(), p.size, p.cap), f"units {uf},{ub} fwd")
+    assert_grads_close(grads, O.backward_f64(g, *p.arrays(), p.cap), f"units {uf},{ub} bwd")
