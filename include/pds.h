@@ -61,13 +61,11 @@ typedef enum pds_mode {
  * window (max_pixel_radius) is never altered. */
 typedef struct pds_tuning {
     int32_t tile;          /* square screen tile edge in pixels: 8, 16, 32 or 64 */
-    int32_t seg_photons;   /* photon records per tile segment (one workgroup each) */
+    int32_t seg_photons;   /* photon records per tile segment (one workgroup each), <= 4096 */
     int32_t block_photons; /* photons per binning workgroup */
     int32_t capacity_pct;  /* records reserved per 100 photons (default 200); more
                               records than that run the photon-order path */
-    int32_t unit_fwd;      /* pixels per work unit of the forward tile walk: 2, 4 or 8 */
-    int32_t unit_bwd;      /* pixels per work unit of the backward tile walk: 2, 4 or 8 */
-    int32_t reserved[2];
+    int32_t reserved[4];
 } pds_tuning;
 
 /* Bytes of device workspace pds_forward_f32 needs for this problem in

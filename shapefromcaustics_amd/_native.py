@@ -27,8 +27,7 @@ class PdsTuning(ctypes.Structure):
     """Mirror of ``pds_tuning`` (include/pds.h)."""
 
     _fields_ = [("tile", ctypes.c_int32), ("seg_photons", ctypes.c_int32), ("block_photons", ctypes.c_int32),
-                ("capacity_pct", ctypes.c_int32), ("unit_fwd", ctypes.c_int32), ("unit_bwd", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 2)]
+                ("capacity_pct", ctypes.c_int32), ("reserved", ctypes.c_int32 * 4)]
 
 
 _lib = None

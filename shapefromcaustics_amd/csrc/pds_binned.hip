@@ -42,8 +42,7 @@ constexpr int kMaxBins = 16384;
 constexpr int kMaxBinBlocks = 1024;
 constexpr int kColThreads = 1024;
 constexpr int kTileThreads = 256;
-constexpr int kTileFwdUnit = 4;
-constexpr int kTileBwdUnit = 8;
+constexpr int kMaxSeg = 4096;  // records per segment (LDS sort index is 16-bit)
 constexpr int kDefaultTile = 32;
 constexpr int kDefaultSeg = 2048;
 constexpr double kDefaultCapacity = 2.0;  // records per photon reserved in the workspace
@@ -62,7 +61,6 @@ struct Ctrl {
 
 struct Geom {
     int32_t T, tshift, tiles_x, tiles_y, nbins, G, seg;
-    int32_t ufwd, ubwd;  // pixels per work unit of the tile kernels
     int64_t per_block, cap_records, max_segments;
     bool det;
 };
@@ -97,6 +95,7 @@ bool make_geom(int64_t N, int32_t C, int32_t H, int32_t W, int32_t cap, int32_t 
     g.tiles_y = (H + T - 1) / T;
     g.nbins = C * g.tiles_x * g.tiles_y;
     g.seg = (t && t->seg_photons > 0) ? t->seg_photons : kDefaultSeg;
+    if (g.seg > kMaxSeg) g.seg = kMaxSeg;
     const int64_t bp = (t && t->block_photons > 0) ? t->block_photons : 0;
     int64_t G = bp > 0 ? (N + bp - 1) / bp : (N + 4095) / 4096;
     if (G > kMaxBinBlocks) G = kMaxBinBlocks;
@@ -109,10 +108,6 @@ bool make_geom(int64_t N, int32_t C, int32_t H, int32_t W, int32_t cap, int32_t 
     if (g.cap_records >= ((int64_t)1 << 32)) return false;  // 32-bit record offsets
     g.max_segments = g.nbins + g.cap_records / g.seg + 1;
     g.det = mode == PDS_MODE_DETERMINISTIC;
-    g.ufwd = (t && t->unit_fwd > 0) ? t->unit_fwd : kTileFwdUnit;
-    g.ubwd = (t && t->unit_bwd > 0) ? t->unit_bwd : kTileBwdUnit;
-    if (g.ufwd != 2 && g.ufwd != 4 && g.ufwd != 8) return false;
-    if (g.ubwd != 2 && g.ubwd != 4 && g.ubwd != 8) return false;
     return true;
 }
 
@@ -447,41 +442,115 @@ __device__ __forceinline__ float from_fixed(unsigned long long v, int e) {
     return (float)ldexp((double)(long long)v, e);
 }
 
-// Decode record j for the tile with origin (X0, Y0) and last pixel (X1, Y1)
-// into a flattened-walk record: box = window n tile n ellipse bounding box,
-// LDS tile index of its origin in r.base.
-template <int U>
-__device__ __forceinline__ void decode_record(const uint4* __restrict__ recs, size_t j, int X0, int Y0, int X1, int Y1,
-                                              int half, int T, FlatRec& r) {
+// One record decoded for the tile with origin (X0, Y0) and last pixel (X1, Y1):
+// its footprint box = window n tile n ellipse bounding box.
+struct TileRec {
+    Frame f;            // ix, iy absolute
+    float e;
+    int32_t lx0, ly0;   // box origin in tile coordinates
+    int32_t kx0, ky0;   // box origin relative to (ix, iy)
+    int32_t bw, area;   // box width and area (0: empty)
+};
+
+__device__ __forceinline__ TileRec decode_tile_rec(const uint4* __restrict__ recs, size_t j, int X0, int Y0, int X1,
+                                                   int Y1, int half) {
+    TileRec t;
     const uint4 q0 = recs[2 * j], q1 = recs[2 * j + 1];
     const uint32_t pk = q0.w;
     const int ix = X0 + half + (int)(int8_t)(pk & 0xff);
     const int iy = Y0 + half + (int)(int8_t)((pk >> 8) & 0xff);
     const int rx = (pk >> 16) & 63, ry = (pk >> 22) & 63;
     const int pxc = ix + (int)((pk >> 28) & 3) - 1, pyc = iy + (int)(pk >> 30) - 1;
-    r.f.ix = ix;
-    r.f.iy = iy;
-    r.f.fx = __uint_as_float(q0.x);
-    r.f.fy = __uint_as_float(q0.y);
-    r.e = __uint_as_float(q0.z);
-    r.f.a00 = __uint_as_float(q1.x);
-    r.f.a01 = __uint_as_float(q1.y);
-    r.f.a10 = __uint_as_float(q1.z);
-    r.f.a11 = __uint_as_float(q1.w);
+    t.f.ix = ix;
+    t.f.iy = iy;
+    t.f.fx = __uint_as_float(q0.x);
+    t.f.fy = __uint_as_float(q0.y);
+    t.e = __uint_as_float(q0.z);
+    t.f.a00 = __uint_as_float(q1.x);
+    t.f.a01 = __uint_as_float(q1.y);
+    t.f.a10 = __uint_as_float(q1.z);
+    t.f.a11 = __uint_as_float(q1.w);
     int x0 = max(max(pxc - rx, X0), 0), x1 = min(pxc + rx, X1);
     int y0 = max(max(pyc - ry, Y0), 0), y1 = min(pyc + ry, Y1);
-    clip_to_ellipse(r.f, x0, x1, y0, y1);
-    if (x0 <= x1 && y0 <= y1) set_box<U>(r, x0, y0, x1 - x0 + 1, y1 - y0 + 1, (y0 - Y0) * T + (x0 - X0));
+    clip_to_ellipse(t.f, x0, x1, y0, y1);
+    const bool any = x0 <= x1 && y0 <= y1;
+    t.bw = any ? x1 - x0 + 1 : 1;
+    t.area = any ? t.bw * (y1 - y0 + 1) : 0;
+    t.lx0 = x0 - X0;
+    t.ly0 = y0 - Y0;
+    t.kx0 = x0 - ix;
+    t.ky0 = y0 - iy;
+    return t;
 }
 
-template <bool DET, int U>
+// Counting sort of a segment's records by footprint area (clamped to 255), so
+// that the 64 records a wavefront walks side by side, one per lane, have
+// nearly the same number of pixels.
+struct SegSort {
+    uint16_t order[kMaxSeg];
+    uint8_t key[kMaxSeg];
+    uint32_t hist[256];
+    uint32_t wsum[kTileThreads / 64];
+};
+
+__device__ __forceinline__ void sort_segment(SegSort& S, const uint4* __restrict__ recs, const int4& d, int X0, int Y0,
+                                             int X1, int Y1, int half) {
+    static_assert(kTileThreads == 256, "one histogram bin per thread");
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    S.hist[tid] = 0;
+    __syncthreads();
+    for (int j = tid; j < d.z; j += kTileThreads) {
+        const TileRec t = decode_tile_rec(recs, (size_t)d.y + j, X0, Y0, X1, Y1, half);
+        const uint32_t key = (uint32_t)min(t.area, 255);
+        S.key[j] = (uint8_t)key;
+        atomicAdd(&S.hist[key], 1u);
+    }
+    __syncthreads();
+    const uint32_t v = S.hist[tid];
+    const uint32_t incl = (uint32_t)wave_inclusive_sum((int64_t)v);
+    if (lane == 63) S.wsum[w] = incl;
+    __syncthreads();
+    uint32_t excl = incl - v;
+    for (int k = 0; k < w; ++k) excl += S.wsum[k];
+    S.hist[tid] = excl;
+    __syncthreads();
+    for (int j = tid; j < d.z; j += kTileThreads) {
+        const uint32_t pos = atomicAdd(&S.hist[S.key[j]], 1u);
+        S.order[pos] = (uint16_t)j;
+    }
+    __syncthreads();
+}
+
+// Segment geometry: channel and tile of the segment's bin.
+struct SegTile {
+    int c, X0, Y0, X1, Y1;
+};
+
+__device__ __forceinline__ SegTile seg_tile(const Geom& g, const ImageArgs& I, int bin) {
+    SegTile st;
+    const int tpc = g.tiles_x * g.tiles_y;
+    st.c = bin / tpc;
+    const int tt = bin - st.c * tpc;
+    const int ty = tt / g.tiles_x, tx = tt - ty * g.tiles_x;
+    st.X0 = tx << g.tshift;
+    st.Y0 = ty << g.tshift;
+    st.X1 = min(st.X0 + g.T, I.W) - 1;
+    st.Y1 = min(st.Y0 + g.T, I.H) - 1;
+    return st;
+}
+
+// ---------------------------------------------------------------------------
+// 7. tile (forward): one workgroup per segment; each lane splats whole
+//    records (area-sorted) into the LDS tile, which is then flushed.
+// ---------------------------------------------------------------------------
+template <bool DET>
 __global__ __launch_bounds__(kTileThreads) void bin_tile_kernel(Geom g, ImageArgs I, const int4* __restrict__ segs,
                                                                 const uint4* __restrict__ recs, const Ctrl* ctrl,
                                                                 float* __restrict__ out,
                                                                 unsigned long long* __restrict__ acc64) {
     using Acc = typename std::conditional<DET, unsigned long long, float>::type;
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ WaveRecs wr[kTileThreads / 64];
+    __shared__ SegSort S;
     Acc* tile = reinterpret_cast<Acc*>(smem);
     const uint32_t s = blockIdx.x;
     if (s >= ctrl->nseg) return;
@@ -489,50 +558,42 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_kernel(Geom g, ImageArg
     const int fe = DET ? ctrl->fx_exp : 0;
     const int T = g.T, TT = T * T;
     for (int k = threadIdx.x; k < TT; k += kTileThreads) tile[k] = Acc(0);
-    const int tpc = g.tiles_x * g.tiles_y;
-    const int c = d.x / tpc, tt = d.x - c * tpc;
-    const int ty = tt / g.tiles_x, tx = tt - ty * g.tiles_x;
-    const int X0 = tx << g.tshift, Y0 = ty << g.tshift;
-    const int X1 = min(X0 + T, I.W) - 1, Y1 = min(Y0 + T, I.H) - 1;
+    const SegTile st = seg_tile(g, I, d.x);
     const int half = T >> 1;
-    __syncthreads();
-
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    WaveRecs& R = wr[w];
-    R.mark[lane] = -1;
-    for (int b0 = w * 64; b0 < d.z; b0 += kTileThreads) {
-        FlatRec r;
-        clear_rec(r);
-        if (b0 + lane < d.z) decode_record<U>(recs, (size_t)d.y + b0 + lane, X0, Y0, X1, Y1, half, T, r);
-        const int64_t total = publish_recs(R, r, lane);
-        walk_units<U>(R, r, lane, total, [&](const FlatItem& it) {
-            const UnitPixels<U> px(it, T);
-#pragma unroll
-            for (int k = 0; k < U; ++k) {
-                if (k < it.n) {
-                    float u, v;
-                    const float val = silverman(eval_q(it.f, px.dx[k], px.dy[k], u, v)) * it.e;
-                    if (val > 0.0f) {
-                        if constexpr (DET) {
-                            const unsigned long long fxv = to_fixed(val, fe);
-                            if (fxv >> 62)
-                                atomicOr(&tile[px.pix[k]], fxv);
-                            else
-                                atomicAdd(&tile[px.pix[k]], fxv);
-                        } else {
-                            atomicAdd(&tile[px.pix[k]], val);
-                        }
-                    }
+    sort_segment(S, recs, d, st.X0, st.Y0, st.X1, st.Y1, half);
+    for (int pos = threadIdx.x; pos < d.z; pos += kTileThreads) {
+        const TileRec t = decode_tile_rec(recs, (size_t)d.y + S.order[pos], st.X0, st.Y0, st.X1, st.Y1, half);
+        int row = 0, col = 0;
+        float dy = (float)t.ky0 - t.f.fy;
+        for (int p = 0; p < t.area; ++p) {
+            const float dx = (float)(t.kx0 + col) - t.f.fx;
+            float u, v;
+            const float val = silverman(eval_q(t.f, dx, dy, u, v)) * t.e;
+            if (val > 0.0f) {
+                const int idx = (t.ly0 + row) * T + t.lx0 + col;
+                if constexpr (DET) {
+                    const unsigned long long fxv = to_fixed(val, fe);
+                    if (fxv >> 62)
+                        atomicOr(&tile[idx], fxv);
+                    else
+                        atomicAdd(&tile[idx], fxv);
+                } else {
+                    atomicAdd(&tile[idx], val);
                 }
             }
-        });
+            if (++col == t.bw) {
+                col = 0;
+                ++row;
+                dy = (float)(t.ky0 + row) - t.f.fy;
+            }
+        }
     }
     __syncthreads();
     const bool split = (d.w & 1) != 0;
     for (int k = threadIdx.x; k < TT; k += kTileThreads) {
-        const int px = X0 + (k & (T - 1)), py = Y0 + (k >> g.tshift);
+        const int px = st.X0 + (k & (T - 1)), py = st.Y0 + (k >> g.tshift);
         if (px >= I.W || py >= I.H) continue;
-        const int64_t o = ((int64_t)c * I.H + py) * I.W + px;
+        const int64_t o = ((int64_t)st.c * I.H + py) * I.W + px;
         const Acc v = tile[k];
         if constexpr (DET) {
             if (!split)
@@ -576,72 +637,49 @@ __global__ __launch_bounds__(kTileThreads) void bin_convert_split_kernel(Geom g,
 
 // ---------------------------------------------------------------------------
 // backward 1: tile adjoint.  The segment's gradient tile is staged in LDS; each
-// record's seven sums over its pixels in the tile are reduced in LDS and
-// stored, in the reference's units, at the record's slot (coalesced).
+// lane walks whole records (area-sorted), sums their seven terms in
+// registers and stores them, in the reference's units, at the record's slot.
 // ---------------------------------------------------------------------------
-template <int U>
 __global__ __launch_bounds__(kTileThreads) void bin_tile_bwd_kernel(Geom g, ImageArgs I, const int4* __restrict__ segs,
                                                                     const uint4* __restrict__ recs, const Ctrl* ctrl,
                                                                     const float* __restrict__ grad,
                                                                     float4* __restrict__ partials) {
     extern __shared__ __align__(16) float gt[];  // T * T upstream gradient tile
-    __shared__ WaveRecs wr[kTileThreads / 64];
-    __shared__ float acc[kTileThreads / 64][7][64];
+    __shared__ SegSort S;
     const uint32_t s = blockIdx.x;
     if (s >= ctrl->nseg) return;
     const int4 d = segs[s];
     const int T = g.T, TT = T * T;
-    const int tpc = g.tiles_x * g.tiles_y;
-    const int c = d.x / tpc, tt = d.x - c * tpc;
-    const int ty = tt / g.tiles_x, tx = tt - ty * g.tiles_x;
-    const int X0 = tx << g.tshift, Y0 = ty << g.tshift;
-    const int X1 = min(X0 + T, I.W) - 1, Y1 = min(Y0 + T, I.H) - 1;
-    const int half = T >> 1;
+    const SegTile st = seg_tile(g, I, d.x);
     for (int k = threadIdx.x; k < TT; k += kTileThreads) {
-        const int px = X0 + (k & (T - 1)), py = Y0 + (k >> g.tshift);
-        gt[k] = (px < I.W && py < I.H) ? grad[((int64_t)c * I.H + py) * I.W + px] : 0.0f;
+        const int px = st.X0 + (k & (T - 1)), py = st.Y0 + (k >> g.tshift);
+        gt[k] = (px < I.W && py < I.H) ? grad[((int64_t)st.c * I.H + py) * I.W + px] : 0.0f;
     }
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    WaveRecs& R = wr[w];
-    R.mark[lane] = -1;
-#pragma unroll
-    for (int k = 0; k < 7; ++k) acc[w][k][lane] = 0.f;
-    __syncthreads();
+    const int half = T >> 1;
+    sort_segment(S, recs, d, st.X0, st.Y0, st.X1, st.Y1, half);
     const float hw = 0.5f * I.fw, hh = 0.5f * I.fh, iw = 2.0f / I.fw, ih = 2.0f / I.fh;
-    for (int b0 = w * 64; b0 < d.z; b0 += kTileThreads) {
-        FlatRec r;
-        clear_rec(r);
-        const size_t j = (size_t)d.y + b0 + lane;
-        if (b0 + lane < d.z) decode_record<U>(recs, j, X0, Y0, X1, Y1, half, T, r);
-        const int64_t total = publish_recs(R, r, lane);
-        walk_units<U>(R, r, lane, total, [&](const FlatItem& it) {
-            const UnitPixels<U> px(it, T);
-            float a[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int k = 0; k < U; ++k) {
-                if (k < it.n) {
-                    float u, v;
-                    const float q = eval_q(it.f, px.dx[k], px.dy[k], u, v);
-                    const float gg = gt[px.pix[k]];
-                    const float sc = dk_over_r(q, it.e * gg);
-                    accumulate_terms(a, q, gg, sc, u, v, px.dx[k], px.dy[k], it.f);
-                }
+    for (int pos = threadIdx.x; pos < d.z; pos += kTileThreads) {
+        const int j = S.order[pos];
+        const TileRec t = decode_tile_rec(recs, (size_t)d.y + j, st.X0, st.Y0, st.X1, st.Y1, half);
+        float a[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        int row = 0, col = 0;
+        float dy = (float)t.ky0 - t.f.fy;
+        for (int p = 0; p < t.area; ++p) {
+            const float dx = (float)(t.kx0 + col) - t.f.fx;
+            float u, v;
+            const float q = eval_q(t.f, dx, dy, u, v);
+            const float gg = gt[(t.ly0 + row) * T + t.lx0 + col];
+            const float sc = dk_over_r(q, t.e * gg);
+            accumulate_terms(a, q, gg, sc, u, v, dx, dy, t.f);
+            if (++col == t.bw) {
+                col = 0;
+                ++row;
+                dy = (float)(t.ky0 + row) - t.f.fy;
             }
-#pragma unroll
-            for (int k = 0; k < 7; ++k) atomicAdd(&acc[w][k][it.rec], a[k]);
-        });
-        wave_lds_fence();
-        float t7[7];
-#pragma unroll
-        for (int k = 0; k < 7; ++k) {
-            t7[k] = acc[w][k][lane];
-            acc[w][k][lane] = 0.f;
         }
-        if (b0 + lane < d.z) {
-            partials[2 * j] = make_float4(t7[0], t7[1] * hw, t7[2] * hh, t7[3] * iw);
-            partials[2 * j + 1] = make_float4(t7[4] * ih, t7[5] * iw, t7[6] * ih, 0.0f);
-        }
-        wave_lds_fence();
+        const size_t slot = (size_t)d.y + j;
+        partials[2 * slot] = make_float4(a[0], a[1] * hw, a[2] * hh, a[3] * iw);
+        partials[2 * slot + 1] = make_float4(a[4] * ih, a[5] * iw, a[6] * ih, 0.0f);
     }
 }
 
@@ -889,17 +927,8 @@ void launch_build(const PhotonArgs& P, const ImageArgs& I, const Geom& g, const 
 
 template <bool DET>
 void launch_tile_fwd(const Geom& g, const ImageArgs& I, const Buffers& B, float* out, size_t lds, hipStream_t s) {
-    const dim3 grid((unsigned)g.max_segments), block(kTileThreads);
-#define PDS_TILE_FWD(U)                                                                                           \
-    hipLaunchKernelGGL((bin_tile_kernel<DET, U>), grid, block, lds, s, g, I, (const int4*)B.segs,                 \
-                       (const uint4*)B.recs, (const Ctrl*)B.ctrl, out, B.acc64)
-    if (g.ufwd == 2)
-        PDS_TILE_FWD(2);
-    else if (g.ufwd == 8)
-        PDS_TILE_FWD(8);
-    else
-        PDS_TILE_FWD(4);
-#undef PDS_TILE_FWD
+    hipLaunchKernelGGL(bin_tile_kernel<DET>, dim3((unsigned)g.max_segments), dim3(kTileThreads), lds, s, g, I,
+                       (const int4*)B.segs, (const uint4*)B.recs, (const Ctrl*)B.ctrl, out, B.acc64);
 }
 
 // zero split tiles -> tiles -> (convert) ; photon-order path on overflow
@@ -935,16 +964,8 @@ void launch_backward_tiles(const float* grad, const PhotonArgs& P, const ImageAr
                            const Buffers& B, const GradArgs& G, hipStream_t s) {
     const unsigned nsegs = (unsigned)g.max_segments;
     const size_t tile_lds = (size_t)g.T * g.T * 4;
-#define PDS_TILE_BWD(U)                                                                                           \
-    hipLaunchKernelGGL((bin_tile_bwd_kernel<U>), dim3(nsegs), dim3(kTileThreads), tile_lds, s, g, I,               \
-                       (const int4*)B.segs, (const uint4*)B.recs, (const Ctrl*)B.ctrl, grad, B.partials)
-    if (g.ubwd == 2)
-        PDS_TILE_BWD(2);
-    else if (g.ubwd == 4)
-        PDS_TILE_BWD(4);
-    else
-        PDS_TILE_BWD(8);
-#undef PDS_TILE_BWD
+hipLaunchKernelGGL(bin_tile_bwd_kernel, dim3(nsegs), dim3(kTileThreads), tile_lds, s, g, I, (const int4*)B.segs,
+                       (const uint4*)B.recs, (const Ctrl*)B.ctrl, grad, B.partials);
     int64_t blocks = (P.N + 255) / 256;
     if (blocks > 8192) blocks = 8192;
     if (blocks < 1) blocks = 1;

@@ -288,11 +288,11 @@ def test_backward_repeatable():
             assert torch.equal(a, b)
 
 
-@pytest.mark.parametrize("uf,ub", [(2, 2), (4, 4), (8, 8), (2, 8)])
-def test_unit_sizes(uf, ub):
-    p = WL.caustic(1 << 16, (3, 192, 160), 12, seed=uf * 10 + ub)
+@pytest.mark.parametrize("seg", [1, 7, 64, 4096])
+def test_segment_sizes(seg):
+    # segments of 1..4096 records: hot tiles split over many workgroups
+    p = WL.caustic(1 << 16, (3, 192, 160), 12, seed=seg)
     g = WL.upstream_grad(p.size, seed=3)
-    tun = {"unit_fwd": uf, "unit_bwd": ub, "tile": 16}
-    img, grads = _run(p, "auto", g, tuning=tun)
-    assert_image_close(img, O.forward_f64(*p.arrays(), p.size, p.cap), f"units {uf},{ub} fwd")
-    assert_grads_close(grads, O.backward_f64(g, *p.arrays(), p.cap), f"units {uf},{ub} bwd")
+    img, grads = _run(p, "auto", g, tuning={"seg_photons": seg, "tile": 16})
+    assert_image_close(img, O.forward_f64(*p.arrays(), p.size, p.cap), f"seg {seg} fwd")
+    assert_grads_close(grads, O.backward_f64(g, *p.arrays(), p.cap), f"seg {seg} bwd")
