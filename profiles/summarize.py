@@ -24,7 +24,7 @@ def main(path):
             d[r[ki]].append((r[ei] - r[si]) / 1e3)
         print("| kernel | calls | avg us | total us |\n|---|---:|---:|---:|")
         for k, v in sorted(d.items(), key=lambda kv: -sum(kv[1])):
-            name = k.split("(")[0].replace("(anonymous namespace)::", "")
+            name = k.replace("(anonymous namespace)::", "").split("(")[0]
             print(f"| `{name}` | {len(v)} | {sum(v) / len(v):.1f} | {sum(v):.1f} |")
     if "counters" in views:
         cols = [r[1] for r in cur.execute("pragma table_info(counters)")]
@@ -37,7 +37,7 @@ def main(path):
             nd[r[ki]].add(r[di])
         print()
         for k, dd in agg.items():
-            name = k.split("(")[0].replace("(anonymous namespace)::", "")
+            name = k.replace("(anonymous namespace)::", "").split("(")[0]
             n = len(nd[k])
             print(f"\n`{name}` ({n} dispatches, per-dispatch averages)\n\n| counter | value |\n|---|---:|")
             for c, v in sorted(dd.items()):
