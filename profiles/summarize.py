@@ -26,13 +26,18 @@ def main(path):
         for k, v in sorted(d.items(), key=lambda kv: -sum(kv[1])):
             name = k.replace("(anonymous namespace)::", "").split("(")[0]
             print(f"| `{name}` | {len(v)} | {sum(v) / len(v):.1f} | {sum(v):.1f} |")
-    if "counters" in views:
-        cols = [r[1] for r in cur.execute("pragma table_info(counters)")]
-        ki, ci, vi = cols.index("kernel_name"), cols.index("counter_name"), cols.index("value")
-        di = cols.index("dispatch_id")
+    view = next((v for v in ("counters", "pmc_events") if v in views), None)
+    if view is not None:
+        cols = [r[1] for r in cur.execute(f"pragma table_info({view})")]
+
+        def col(*names):
+            return next(cols.index(n) for n in names if n in cols)
+
+        ki, ci, vi = col("kernel_name", "name"), col("counter_name"), col("value", "counter_value")
+        di = col("dispatch_id", "id")
         agg = defaultdict(lambda: defaultdict(float))
         nd = defaultdict(set)
-        for r in cur.execute("select * from counters"):
+        for r in cur.execute(f"select * from {view}"):
             agg[r[ki]][r[ci]] += r[vi]
             nd[r[ki]].add(r[di])
         print()

@@ -564,13 +564,13 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_kernel(Geom g, ImageArg
     for (int pos = threadIdx.x; pos < d.z; pos += kTileThreads) {
         const TileRec t = decode_tile_rec(recs, (size_t)d.y + S.order[pos], st.X0, st.Y0, st.X1, st.Y1, half);
         int row = 0, col = 0;
+        int idx = (t.ly0 << g.tshift) + t.lx0;
         float dy = (float)t.ky0 - t.f.fy;
-        for (int p = 0; p < t.area; ++p) {
+        for (int p = 0; p < t.area; ++p, ++idx) {
             const float dx = (float)(t.kx0 + col) - t.f.fx;
             float u, v;
             const float val = silverman(eval_q(t.f, dx, dy, u, v)) * t.e;
             if (val > 0.0f) {
-                const int idx = (t.ly0 + row) * T + t.lx0 + col;
                 if constexpr (DET) {
                     const unsigned long long fxv = to_fixed(val, fe);
                     if (fxv >> 62)
@@ -584,6 +584,7 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_kernel(Geom g, ImageArg
             if (++col == t.bw) {
                 col = 0;
                 ++row;
+                idx += T - t.bw;
                 dy = (float)(t.ky0 + row) - t.f.fy;
             }
         }
@@ -661,25 +662,47 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_bwd_kernel(Geom g, Imag
     for (int pos = threadIdx.x; pos < d.z; pos += kTileThreads) {
         const int j = S.order[pos];
         const TileRec t = decode_tile_rec(recs, (size_t)d.y + j, st.X0, st.Y0, st.X1, st.Y1, half);
-        float a[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        int row = 0, col = 0;
+        // Per pixel, with w = (1 - q) inside the footprint and 0 outside,
+        // accumulate  m0 = sum w^2 g  and the moments of  w g  over the pixel
+        // offsets (dx, dy):  m1 = sum w g dx, m2 = sum w g dy, m3 = sum w g dx^2,
+        // m4 = sum w g dx dy, m5 = sum w g dy^2.  With s = dk / r = c w g,
+        // c = -(12/pi) E, and (u, v) = A (dx, dy), the reference's sums
+        // (photon_differentials.cu:195-204) are linear in these moments.
+        float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f, m4 = 0.f, m5 = 0.f;
+        int col = 0, row = 0;
+        int li = (t.ly0 << g.tshift) + t.lx0;
         float dy = (float)t.ky0 - t.f.fy;
         for (int p = 0; p < t.area; ++p) {
             const float dx = (float)(t.kx0 + col) - t.f.fx;
             float u, v;
             const float q = eval_q(t.f, dx, dy, u, v);
-            const float gg = gt[(t.ly0 + row) * T + t.lx0 + col];
-            const float sc = dk_over_r(q, t.e * gg);
-            accumulate_terms(a, q, gg, sc, u, v, dx, dy, t.f);
+            const float w = q < 1.0f ? 1.0f - q : 0.0f;
+            const float wg = w * gt[li];
+            m0 = fmaf(w, wg, m0);
+            const float px = wg * dx, py = wg * dy;
+            m1 += px;
+            m2 += py;
+            m3 = fmaf(px, dx, m3);
+            m4 = fmaf(px, dy, m4);
+            m5 = fmaf(py, dy, m5);
+            ++li;
             if (++col == t.bw) {
                 col = 0;
                 ++row;
+                li += T - t.bw;
                 dy = (float)(t.ky0 + row) - t.f.fy;
             }
         }
+        const Frame& f = t.f;
+        const float c = -kTwelveOverPi * t.e;
+        const float su = c * fmaf(f.a00, m1, f.a01 * m2);  // sum s u
+        const float sv = c * fmaf(f.a10, m1, f.a11 * m2);  // sum s v
+        const float sudx = c * fmaf(f.a00, m3, f.a01 * m4), sudy = c * fmaf(f.a00, m4, f.a01 * m5);
+        const float svdx = c * fmaf(f.a10, m3, f.a11 * m4), svdy = c * fmaf(f.a10, m4, f.a11 * m5);
+        const float gx0 = -fmaf(f.a00, su, f.a10 * sv), gx1 = -fmaf(f.a01, su, f.a11 * sv);
         const size_t slot = (size_t)d.y + j;
-        partials[2 * slot] = make_float4(a[0], a[1] * hw, a[2] * hh, a[3] * iw);
-        partials[2 * slot + 1] = make_float4(a[4] * ih, a[5] * iw, a[6] * ih, 0.0f);
+        partials[2 * slot] = make_float4(kThreeOverPi * m0, gx0 * hw, gx1 * hh, sudx * iw);
+        partials[2 * slot + 1] = make_float4(sudy * ih, svdx * iw, svdy * ih, 0.0f);
     }
 }
 
