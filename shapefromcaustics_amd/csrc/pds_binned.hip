@@ -6,17 +6,18 @@
 // photons' (channel, screen tile) overlaps into bins and accumulates each tile
 // in LDS, flushing it to HBM with coalesced stores:
 //
-//   1. count     per-block LDS histogram of bin overlaps           -> hist[G][nbins]
-//   2. colsum    per-bin totals over blocks                         -> totals[nbins]
-//   3. plan      exclusive scan of totals, segment list (<= seg records each),
+//   0. init      zero the per-bin totals and the control block
+//   1. count     per-block LDS histogram of bin overlaps; one device atomic per
+//                (block, non-empty bin) on totals[bin] reserves the block's
+//                range inside the bin -> hist[G][nbins] = offset within bin
+//   2. plan      exclusive scan of totals, segment list (<= seg records each),
 //                capacity check, fixed-point scale for the deterministic mode
-//   4. offsets   exclusive per-(block, bin) record offsets, in place in hist
-//   5. scatter   each block re-walks its photons and writes one 32-byte record
-//                per overlap at slot = atomicAdd_LDS(cursor[bin], 1)
-//   6. zero      clears the tiles of bins split over several segments
-//   7. tile      one workgroup per segment: flattened walk of its records,
+//   3. scatter   each block re-walks its photons and writes one 32-byte record
+//                per overlap at slot = binoff + atomicAdd_LDS(cursor[bin], 1);
+//                then clears the tiles of bins split over several segments
+//   4. tile      one workgroup per segment: area-sorted walk of its records,
 //                LDS accumulation, flush (store, or atomic add when split)
-//   8. convert   deterministic mode: int64 fixed point -> float for split bins
+//   5. convert   deterministic mode: int64 fixed point -> float for split bins
 //
 // Atomic mode accumulates float in LDS (summation order varies run to run).
 // Deterministic mode accumulates every contribution as a 64-bit fixed-point
@@ -24,9 +25,10 @@
 // resolution of any pixel), so the sum is exact and order-independent: the
 // output is bitwise reproducible.  +inf contributions set bit 62.
 //
-// If the records exceed the workspace capacity, every binned kernel turns
-// into a no-op and the photon-order path (flattened walk, global atomics /
-// global int64 atomics) produces the result instead; a device flag chooses.
+// If the records exceed the workspace capacity, the same launches switch to
+// the photon-order path (scatter zero-fills, tile runs the flattened walk with
+// global atomics / global int64 atomics, convert converts the whole image);
+// a device flag written by plan chooses, so the host never synchronises.
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
@@ -40,11 +42,10 @@ namespace {
 constexpr int kBinThreads = 512;
 constexpr int kMaxBins = 16384;
 constexpr int kMaxBinBlocks = 1024;
-constexpr int kColThreads = 1024;
 constexpr int kTileThreads = 256;
 constexpr int kMaxSeg = 4096;  // records per segment (LDS sort index is 16-bit)
-constexpr int kDefaultTile = 32;
-constexpr int kDefaultSeg = 2048;
+constexpr int kMinBins = 256;
+constexpr int kDefaultSeg = 1024;
 constexpr double kDefaultCapacity = 2.0;  // records per photon reserved in the workspace
 constexpr int kMaxRecsPerPhoton = 4;      // backward: photons with more records are recomputed directly
 constexpr uint8_t kSpecial = 255;         // class of photons the backward recomputes directly
@@ -66,7 +67,7 @@ struct Geom {
 };
 
 struct Layout {
-    size_t hist, totals, binoff, segs, ctrl, recs, slots, info, partials, acc64, end;
+    size_t hist, totals, binoff, emax, segs, ctrl, recs, slots, info, partials, acc64, end;
 };
 
 inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -77,11 +78,22 @@ int ilog2(int v) {
     return s;
 }
 
-// Tile edge: the tuned one if given, else 32, raised until the bins fit the
-// LDS histogram.  0 means the binned path cannot run (photon order instead).
+// Default tile edge: the largest of 64, 32, 16 that still gives at least
+// kMinBins bins (a few per CU pair), since larger tiles amortise the flush and
+// keep more of a footprint in one tile; 8 for images too small for that.
+int default_tile(int32_t C, int32_t H, int32_t W) {
+    for (int T = 64; T >= 16; T >>= 1) {
+        const int64_t bins = (int64_t)C * ((W + T - 1) / T) * ((H + T - 1) / T);
+        if (bins >= kMinBins) return T;
+    }
+    return 8;
+}
+
+// Tile edge: the tuned one if given, else the default, raised until the bins
+// fit the LDS histogram.  false: the binned path cannot run (photon order).
 bool make_geom(int64_t N, int32_t C, int32_t H, int32_t W, int32_t cap, int32_t mode, const pds_tuning* t, Geom& g) {
     if (cap > 63) return false;  // packed record holds window radii < 64
-    int T = (t && t->tile > 0) ? t->tile : kDefaultTile;
+    int T = (t && t->tile > 0) ? t->tile : default_tile(C, H, W);
     if (T != 8 && T != 16 && T != 32 && T != 64) return false;
     for (;;) {
         const int64_t tx = (W + T - 1) / T, ty = (H + T - 1) / T;
@@ -122,6 +134,8 @@ Layout make_layout(const Geom& g, int64_t N, int32_t C, int32_t H, int32_t W, bo
     o = align256(o + sizeof(uint32_t) * (size_t)g.nbins);
     L.binoff = o;
     o = align256(o + sizeof(uint32_t) * (size_t)g.nbins);
+    L.emax = o;
+    o = align256(o + sizeof(float) * (size_t)g.G);
     L.segs = o;
     o = align256(o + sizeof(int4) * (size_t)g.max_segments);
     L.ctrl = o;
@@ -214,10 +228,32 @@ __device__ __forceinline__ uint32_t pack_rec(int32_t dcx, int32_t dcy, int32_t r
 }
 
 // ---------------------------------------------------------------------------
-// 1. count
+// 0. init: totals and the control block (no memset, so the sequence can be
+//    captured in a graph)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void bin_init_kernel(Geom g, uint32_t* __restrict__ totals, Ctrl* ctrl) {
+    for (int k = threadIdx.x; k < g.nbins; k += 1024) totals[k] = 0u;
+    if (threadIdx.x == 0) {
+        ctrl->nseg = 0;
+        ctrl->overflow = 0;
+        ctrl->max_e_bits = 0;
+        ctrl->fx_exp = 0;
+        ctrl->total = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 1. count: LDS histogram of the block's overlaps, then one device atomic per
+//    non-empty bin reserves the block's range inside that bin (the ranges'
+//    order among blocks varies run to run; nothing downstream depends on it:
+//    the atomic mode's sums are unordered anyway, the deterministic mode's
+//    fixed-point sums are exact, and every record's gradient partial is
+//    computed on its own).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBinThreads) void bin_count_kernel(PhotonArgs P, ImageArgs I, Geom g,
-                                                                uint32_t* __restrict__ hist, Ctrl* ctrl) {
+                                                                uint32_t* __restrict__ hist,
+                                                                uint32_t* __restrict__ totals,
+                                                                float* __restrict__ emax_blk) {
     extern __shared__ uint32_t lh[];  // nbins entries
     __shared__ float wmax[kBinThreads / 64];
     for (int k = threadIdx.x; k < g.nbins; k += kBinThreads) lh[k] = 0;
@@ -240,67 +276,40 @@ __global__ __launch_bounds__(kBinThreads) void bin_count_kernel(PhotonArgs P, Im
     }
     __syncthreads();
     uint32_t* row = hist + (size_t)blockIdx.x * g.nbins;
-    for (int k = threadIdx.x; k < g.nbins; k += kBinThreads) row[k] = lh[k];
+    for (int k = threadIdx.x; k < g.nbins; k += kBinThreads) {
+        const uint32_t n = lh[k];
+        if (n) row[k] = atomicAdd(&totals[k], n);  // entries of empty bins are never read
+    }
     if (g.det && threadIdx.x == 0) {
         float m = 0.0f;
         for (int w = 0; w < kBinThreads / 64; ++w) m = fmaxf(m, wmax[w]);
-        atomicMax(&ctrl->max_e_bits, __float_as_uint(m));  // positive floats order like their bits
+        emax_blk[blockIdx.x] = m;
     }
 }
 
 // ---------------------------------------------------------------------------
-// 2. per-bin totals over blocks (and 4. offsets): 64 bins per workgroup,
-//    16 wavefronts each own a contiguous range of block rows.
-// ---------------------------------------------------------------------------
-template <bool OFFSETS>
-__global__ __launch_bounds__(kColThreads) void bin_columns_kernel(Geom g, uint32_t* __restrict__ hist,
-                                                                  uint32_t* __restrict__ totals,
-                                                                  const uint32_t* __restrict__ binoff,
-                                                                  const Ctrl* ctrl) {
-    constexpr int WAVES = kColThreads / 64;
-    __shared__ uint32_t part[WAVES][64];
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int bin = blockIdx.x * 64 + lane;
-    const int rows = (g.G + WAVES - 1) / WAVES;
-    const int r0 = w * rows, r1 = min(r0 + rows, g.G);
-    uint32_t s = 0;
-    if (bin < g.nbins)
-        for (int r = r0; r < r1; ++r) s += hist[(size_t)r * g.nbins + bin];
-    part[w][lane] = s;
-    __syncthreads();
-    if (!OFFSETS) {
-        if (w == 0 && bin < g.nbins) {
-            uint32_t t = 0;
-            for (int k = 0; k < WAVES; ++k) t += part[k][lane];
-            totals[bin] = t;
-        }
-        return;
-    }
-    if (ctrl->overflow || bin >= g.nbins) return;
-    uint32_t run = binoff[bin];
-    for (int k = 0; k < w; ++k) run += part[k][lane];
-    for (int r = r0; r < r1; ++r) {
-        uint32_t* p = hist + (size_t)r * g.nbins + bin;
-        const uint32_t v = *p;
-        *p = run;
-        run += v;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// 3. plan: one workgroup
+// 2. plan: one workgroup
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(1024) void bin_plan_kernel(Geom g, int64_t N, const uint32_t* __restrict__ totals,
+                                                        const float* __restrict__ emax_blk,
                                                         uint32_t* __restrict__ binoff, int4* __restrict__ segs,
                                                         Ctrl* ctrl) {
     __shared__ uint64_t wsum[16];
     __shared__ uint32_t wseg[16];
+    __shared__ float wmax[16];
     __shared__ uint64_t base_rec;
     __shared__ uint32_t base_seg;
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     if (tid == 0) {
         base_rec = 0;
         base_seg = 0;
+    }
+    if (g.det) {
+        // maximum finite positive energy over the binning blocks
+        float m = 0.0f;
+        for (int k = tid; k < g.G; k += 1024) m = fmaxf(m, emax_blk[k]);
+        for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+        if (lane == 0) wmax[w] = m;
     }
     __syncthreads();
     // bins in chunks of 1024, one per thread, carried across chunks
@@ -355,7 +364,10 @@ __global__ __launch_bounds__(1024) void bin_plan_kernel(Geom g, int64_t N, const
         ctrl->overflow = over ? 1u : 0u;
         ctrl->nseg = over ? 0u : base_seg;
         // fixed-point quantum: 2^e with N * max(E) * 3/pi <= 2^(e + 61)
-        const float m = __uint_as_float(ctrl->max_e_bits);
+        float m = 0.0f;
+        if (g.det)
+            for (int k = 0; k < 16; ++k) m = fmaxf(m, wmax[k]);
+        ctrl->max_e_bits = __float_as_uint(m);
         int e = 0;
         if (m > 0.0f) {
             const double bound = (double)N * (double)m * 0.9549296585513720;
@@ -366,17 +378,36 @@ __global__ __launch_bounds__(1024) void bin_plan_kernel(Geom g, int64_t N, const
 }
 
 // ---------------------------------------------------------------------------
-// 5. scatter
+// 3. scatter.  On overflow it zero-fills the output (or the int64 image) for
+//    the photon-order path instead.  Afterwards it clears the tiles of split
+//    bins, which the tile kernels then accumulate into atomically.
 // ---------------------------------------------------------------------------
 template <bool KEEP>
 __global__ __launch_bounds__(kBinThreads) void bin_scatter_kernel(PhotonArgs P, ImageArgs I, Geom g,
                                                                   const uint32_t* __restrict__ hist,
+                                                                  const uint32_t* __restrict__ binoff,
+                                                                  const int4* __restrict__ segs,
                                                                   uint4* __restrict__ recs, uint32_t* __restrict__ slots,
-                                                                  uint8_t* __restrict__ info, const Ctrl* ctrl) {
+                                                                  uint8_t* __restrict__ info, const Ctrl* ctrl,
+                                                                  float* __restrict__ out,
+                                                                  unsigned long long* __restrict__ acc64) {
     extern __shared__ uint32_t cur[];  // nbins entries
-    if (ctrl->overflow) return;
+    // the image to clear: none when binning for a backward (no image output)
+    const bool clear = g.det ? acc64 != nullptr : out != nullptr;
+    if (ctrl->overflow) {
+        if (!clear) return;
+        const int64_t n = (int64_t)I.C * I.H * I.W;
+        const int64_t stride = (int64_t)gridDim.x * kBinThreads;
+        for (int64_t k = (int64_t)blockIdx.x * kBinThreads + threadIdx.x; k < n; k += stride) {
+            if (g.det)
+                acc64[k] = 0ull;
+            else
+                out[k] = 0.0f;
+        }
+        return;
+    }
     const uint32_t* row = hist + (size_t)blockIdx.x * g.nbins;
-    for (int k = threadIdx.x; k < g.nbins; k += kBinThreads) cur[k] = row[k];
+    for (int k = threadIdx.x; k < g.nbins; k += kBinThreads) cur[k] = binoff[k] + row[k];
     __syncthreads();
     const int64_t lo = (int64_t)blockIdx.x * g.per_block;
     const int64_t hi = min(lo + g.per_block, P.N);
@@ -405,31 +436,31 @@ __global__ __launch_bounds__(kBinThreads) void bin_scatter_kernel(PhotonArgs P, 
         }
         if (KEEP) info[i] = (b.special || n > kMaxRecsPerPhoton) ? kSpecial : (uint8_t)n;
     }
-}
-
-// ---------------------------------------------------------------------------
-// 6. zero the tiles of split bins (the output in atomic mode, the int64
-//    accumulator in deterministic mode)
-// ---------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(kTileThreads) void bin_zero_split_kernel(Geom g, ImageArgs I, const int4* __restrict__ segs,
-                                                                      const Ctrl* ctrl, T* __restrict__ dst) {
-    const uint32_t s = blockIdx.x;
-    if (s >= ctrl->nseg) return;
-    const int4 d = segs[s];
-    if ((d.w & 3) != 3) return;  // first segment of a split bin only
+    // clear the tiles of split bins (first segment of each such bin)
+    if (!clear) return;
+    const uint32_t nseg = ctrl->nseg;
     const int tpc = g.tiles_x * g.tiles_y;
-    const int c = d.x / tpc, t = d.x - c * tpc;
-    const int ty = t / g.tiles_x, tx = t - ty * g.tiles_x;
-    const int X0 = tx << g.tshift, Y0 = ty << g.tshift;
-    for (int k = threadIdx.x; k < g.T * g.T; k += kTileThreads) {
-        const int px = X0 + (k & (g.T - 1)), py = Y0 + (k >> g.tshift);
-        if (px < I.W && py < I.H) dst[((int64_t)c * I.H + py) * I.W + px] = T(0);
+    for (uint32_t sg = blockIdx.x; sg < nseg; sg += gridDim.x) {
+        const int4 d = segs[sg];
+        if ((d.w & 3) != 3) continue;
+        const int c = d.x / tpc, t = d.x - c * tpc;
+        const int ty = t / g.tiles_x, tx = t - ty * g.tiles_x;
+        const int X0 = tx << g.tshift, Y0 = ty << g.tshift;
+        for (int k = threadIdx.x; k < g.T * g.T; k += kBinThreads) {
+            const int px = X0 + (k & (g.T - 1)), py = Y0 + (k >> g.tshift);
+            if (px < I.W && py < I.H) {
+                const int64_t o = ((int64_t)c * I.H + py) * I.W + px;
+                if (g.det)
+                    acc64[o] = 0ull;
+                else
+                    out[o] = 0.0f;
+            }
+        }
     }
 }
 
 // ---------------------------------------------------------------------------
-// 7. tile
+// tile helpers
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ unsigned long long to_fixed(float val, int e) {
     // val in (0, 2^(e+61)]: quantise to the 2^e grid; +inf sets bit 62
@@ -539,19 +570,69 @@ __device__ __forceinline__ SegTile seg_tile(const Geom& g, const ImageArgs& I, i
     return st;
 }
 
-// ---------------------------------------------------------------------------
-// 7. tile (forward): one workgroup per segment; each lane splats whole
-//    records (area-sorted) into the LDS tile, which is then flushed.
-// ---------------------------------------------------------------------------
+// Photon-order forward of every photon (flattened walk, global atomics or
+// global int64 atomics), grid-stride over 64-photon batches; 256 threads.
 template <bool DET>
-__global__ __launch_bounds__(kTileThreads) void bin_tile_kernel(Geom g, ImageArgs I, const int4* __restrict__ segs,
+__device__ __forceinline__ void flat_splat_all(const PhotonArgs& P, const ImageArgs& I, int fe, WaveRecs* recs,
+                                               float* __restrict__ out, unsigned long long* __restrict__ acc64) {
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    WaveRecs& R = recs[w];
+    R.mark[lane] = -1;
+    const int64_t nbatch = (P.N + 63) >> 6;
+    for (int64_t b = (int64_t)blockIdx.x * 4 + w; b < nbatch; b += (int64_t)gridDim.x * 4) {
+        const int64_t i = (b << 6) + lane;
+        FlatRec r;
+        setup_flat_rec<kFwdUnit>(P, I, i, /*forward=*/true, r);
+        const int64_t total = publish_recs(R, r, lane);
+        walk_units<kFwdUnit>(R, r, lane, total, [&](const FlatItem& it) {
+            if constexpr (DET) {
+                const UnitPixels<kFwdUnit> px(it, I.W);
+#pragma unroll
+                for (int k = 0; k < kFwdUnit; ++k) {
+                    if (k < it.n) {
+                        float u, v;
+                        const float val = silverman(eval_q(it.f, px.dx[k], px.dy[k], u, v)) * it.e;
+                        if (val > 0.0f) {
+                            const unsigned long long fxv = to_fixed(val, fe);
+                            if (fxv >> 62)
+                                atomicOr(&acc64[px.pix[k]], fxv);
+                            else
+                                atomicAdd(&acc64[px.pix[k]], fxv);
+                        }
+                    }
+                }
+            } else {
+                flat_forward_unit<kFwdUnit>(it, I.W, out);
+            }
+        });
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 4. tile (forward): one workgroup per segment; each lane splats whole
+//    records (area-sorted) into the LDS tile, which is then flushed.  On
+//    overflow the workgroups run the photon-order walk instead.
+// ---------------------------------------------------------------------------
+union TileShared {
+    SegSort sort;
+    WaveRecs flat[kTileThreads / 64];
+};
+
+template <bool DET>
+__global__ __launch_bounds__(kTileThreads) void bin_tile_kernel(Geom g, PhotonArgs P, ImageArgs I,
+                                                                const int4* __restrict__ segs,
                                                                 const uint4* __restrict__ recs, const Ctrl* ctrl,
                                                                 float* __restrict__ out,
                                                                 unsigned long long* __restrict__ acc64) {
     using Acc = typename std::conditional<DET, unsigned long long, float>::type;
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ SegSort S;
+    __shared__ TileShared U;
+    SegSort& S = U.sort;
     Acc* tile = reinterpret_cast<Acc*>(smem);
+    if (ctrl->overflow) {
+        flat_splat_all<DET>(P, I, DET ? ctrl->fx_exp : 0, U.flat, out, acc64);
+        return;
+    }
     const uint32_t s = blockIdx.x;
     if (s >= ctrl->nseg) return;
     const int4 d = segs[s];
@@ -611,13 +692,22 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_kernel(Geom g, ImageArg
 }
 
 // ---------------------------------------------------------------------------
-// 8. convert split tiles (deterministic mode)
+// 5. convert split tiles (deterministic mode), or the whole image after the
+//    photon-order path
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kTileThreads) void bin_convert_split_kernel(Geom g, ImageArgs I,
                                                                          const int4* __restrict__ segs,
                                                                          const Ctrl* ctrl,
                                                                          const unsigned long long* __restrict__ acc64,
                                                                          float* __restrict__ out) {
+    if (ctrl->overflow) {
+        const int fe = ctrl->fx_exp;
+        const int64_t n = (int64_t)I.C * I.H * I.W;
+        const int64_t stride = (int64_t)gridDim.x * kTileThreads;
+        for (int64_t k = (int64_t)blockIdx.x * kTileThreads + threadIdx.x; k < n; k += stride)
+            out[k] = from_fixed(acc64[k], fe);
+        return;
+    }
     const uint32_t s = blockIdx.x;
     if (s >= ctrl->nseg) return;
     const int4 d = segs[s];
@@ -708,34 +798,43 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_bwd_kernel(Geom g, Imag
 
 // ---------------------------------------------------------------------------
 // backward 2: finalize in photon order -- sum each photon's record partials
-// (in tile order), or walk a special photon directly; coalesced stores.
+// (in tile order) and store coalesced.  Special photons (non-finite inputs,
+// more than kMaxRecsPerPhoton records) are left to bin_direct_kernel.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void bin_finalize_kernel(const float* __restrict__ grad, PhotonArgs P, ImageArgs I,
-                                                           const Ctrl* ctrl, const uint8_t* __restrict__ info,
+__global__ __launch_bounds__(256) void bin_finalize_kernel(PhotonArgs P, const Ctrl* ctrl,
+                                                           const uint8_t* __restrict__ info,
                                                            const uint32_t* __restrict__ slots,
                                                            const float4* __restrict__ partials, GradArgs G) {
     if (ctrl->overflow) return;
     const int64_t N = P.N;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride) {
-        const uint8_t cls = info[i];
-        if (cls == kSpecial) {
-            float a[7];
-            photon_grad_direct(grad, P, I, i, /*clip_box=*/true, a);
-            store_grads(G, I, N, i, a);
-            continue;
+        const uint32_t cls = info[i];
+        if (cls == kSpecial) continue;
+        // all slot loads first, then all partial loads: up to 8 gathers in flight
+        uint32_t j[kMaxRecsPerPhoton];
+#pragma unroll
+        for (int k = 0; k < kMaxRecsPerPhoton; ++k) j[k] = k < (int)cls ? slots[(size_t)k * N + i] : 0u;
+        float4 p[2 * kMaxRecsPerPhoton];
+#pragma unroll
+        for (int k = 0; k < kMaxRecsPerPhoton; ++k) {
+            if (k < (int)cls) {
+                p[2 * k] = partials[2 * (size_t)j[k]];
+                p[2 * k + 1] = partials[2 * (size_t)j[k] + 1];
+            } else {
+                p[2 * k] = p[2 * k + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
         }
-        float4 s0 = make_float4(0.f, 0.f, 0.f, 0.f), s1 = s0;
-        for (int k = 0; k < (int)cls; ++k) {
-            const size_t j = slots[(size_t)k * N + i];
-            const float4 p0 = partials[2 * j], p1 = partials[2 * j + 1];
-            s0.x += p0.x;
-            s0.y += p0.y;
-            s0.z += p0.z;
-            s0.w += p0.w;
-            s1.x += p1.x;
-            s1.y += p1.y;
-            s1.z += p1.z;
+        float4 s0 = p[0], s1 = p[1];
+#pragma unroll
+        for (int k = 1; k < kMaxRecsPerPhoton; ++k) {
+            s0.x += p[2 * k].x;
+            s0.y += p[2 * k].y;
+            s0.z += p[2 * k].z;
+            s0.w += p[2 * k].w;
+            s1.x += p[2 * k + 1].x;
+            s1.y += p[2 * k + 1].y;
+            s1.z += p[2 * k + 1].z;
         }
         G.gE[i] = s0.x;
         G.gx[i] = s0.y;
@@ -749,12 +848,15 @@ __global__ __launch_bounds__(256) void bin_finalize_kernel(const float* __restri
     }
 }
 
-// capacity overflow: every photon by the direct walk
-__global__ __launch_bounds__(256) void guarded_bwd_direct_kernel(const float* __restrict__ grad, PhotonArgs P,
-                                                                 ImageArgs I, const Ctrl* ctrl, GradArgs G) {
-    if (!ctrl->overflow) return;
+// backward 3: direct serial walk of the special photons, or of every photon
+// when the records overflowed the workspace
+__global__ __launch_bounds__(256) void bin_direct_kernel(const float* __restrict__ grad, PhotonArgs P, ImageArgs I,
+                                                         const Ctrl* ctrl, const uint8_t* __restrict__ info,
+                                                         GradArgs G) {
+    const bool all = ctrl->overflow != 0;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < P.N; i += stride) {
+        if (!all && info[i] != kSpecial) continue;
         float a[7];
         photon_grad_direct(grad, P, I, i, /*clip_box=*/true, a);
         store_grads(G, I, P.N, i, a);
@@ -762,14 +864,8 @@ __global__ __launch_bounds__(256) void guarded_bwd_direct_kernel(const float* __
 }
 
 // ---------------------------------------------------------------------------
-// photon-order fallback when the records exceed the workspace capacity
+// photon-order deterministic forward (problems the binned path cannot take)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void guarded_fill_kernel(const Ctrl* ctrl, float* __restrict__ out, int64_t n) {
-    if (!ctrl->overflow) return;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) out[i] = 0.0f;
-}
-
 __global__ __launch_bounds__(256) void guarded_fill64_kernel(const Ctrl* ctrl, unsigned long long* __restrict__ p,
                                                              int64_t n) {
     if (!ctrl->overflow) return;
@@ -791,38 +887,7 @@ __global__ __launch_bounds__(256) void guarded_flat_kernel(PhotonArgs P, ImageAr
                                                            unsigned long long* __restrict__ acc64) {
     __shared__ WaveRecs recs[4];
     if (!ctrl->overflow) return;
-    const int fe = DET ? ctrl->fx_exp : 0;
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    WaveRecs& R = recs[w];
-    R.mark[lane] = -1;
-    const int64_t nbatch = (P.N + 63) >> 6;
-    for (int64_t b = (int64_t)blockIdx.x * 4 + w; b < nbatch; b += (int64_t)gridDim.x * 4) {
-        const int64_t i = (b << 6) + lane;
-        FlatRec r;
-        setup_flat_rec<kFwdUnit>(P, I, i, /*forward=*/true, r);
-        const int64_t total = publish_recs(R, r, lane);
-        walk_units<kFwdUnit>(R, r, lane, total, [&](const FlatItem& it) {
-            if constexpr (DET) {
-                const UnitPixels<kFwdUnit> px(it, I.W);
-#pragma unroll
-                for (int k = 0; k < kFwdUnit; ++k) {
-                    if (k < it.n) {
-                        float u, v;
-                        const float val = silverman(eval_q(it.f, px.dx[k], px.dy[k], u, v)) * it.e;
-                        if (val > 0.0f) {
-                            const unsigned long long fxv = to_fixed(val, fe);
-                            if (fxv >> 62)
-                                atomicOr(&acc64[px.pix[k]], fxv);
-                            else
-                                atomicAdd(&acc64[px.pix[k]], fxv);
-                        }
-                    }
-                }
-            } else {
-                flat_forward_unit<kFwdUnit>(it, I.W, out);
-            }
-        });
-    }
+    flat_splat_all<DET>(P, I, ctrl->fx_exp, recs, out, acc64);
 }
 
 // maximum finite positive energy of the photons that can contribute
@@ -896,6 +961,7 @@ namespace {
 
 struct Buffers {
     uint32_t *hist, *totals, *binoff, *slots;
+    float* emax;
     int4* segs;
     Ctrl* ctrl;
     uint4* recs;
@@ -910,6 +976,7 @@ Buffers buffers(void* ws, const Layout& L) {
     B.hist = (uint32_t*)(b + L.hist);
     B.totals = (uint32_t*)(b + L.totals);
     B.binoff = (uint32_t*)(b + L.binoff);
+    B.emax = (float*)(b + L.emax);
     B.segs = (int4*)(b + L.segs);
     B.ctrl = (Ctrl*)(b + L.ctrl);
     B.recs = (uint4*)(b + L.recs);
@@ -926,77 +993,55 @@ bool binnable(const ImageArgs& I, int64_t N, int32_t mode, const pds_tuning* t, 
     return I.ell_trivial && make_geom(N, I.C, I.H, I.W, I.cap, mode, t, g);
 }
 
-// count -> columns -> plan -> offsets -> scatter
-void launch_build(const PhotonArgs& P, const ImageArgs& I, const Geom& g, const Buffers& B, bool keep,
+// init -> count -> plan -> scatter
+void launch_build(const PhotonArgs& P, const ImageArgs& I, const Geom& g, const Buffers& B, bool keep, float* out,
                   hipStream_t s) {
-    const unsigned colblocks = (unsigned)((g.nbins + 63) / 64);
     const size_t hist_lds = sizeof(uint32_t) * (size_t)g.nbins;
-    hipLaunchKernelGGL(ctrl_init_kernel, dim3(1), dim3(1), 0, s, B.ctrl);
-    hipLaunchKernelGGL(bin_count_kernel, dim3(g.G), dim3(kBinThreads), hist_lds, s, P, I, g, B.hist, B.ctrl);
-    hipLaunchKernelGGL(bin_columns_kernel<false>, dim3(colblocks), dim3(kColThreads), 0, s, g, B.hist, B.totals,
-                       (const uint32_t*)B.binoff, (const Ctrl*)B.ctrl);
-    hipLaunchKernelGGL(bin_plan_kernel, dim3(1), dim3(1024), 0, s, g, P.N, (const uint32_t*)B.totals, B.binoff,
-                       B.segs, B.ctrl);
-    hipLaunchKernelGGL(bin_columns_kernel<true>, dim3(colblocks), dim3(kColThreads), 0, s, g, B.hist, B.totals,
-                       (const uint32_t*)B.binoff, (const Ctrl*)B.ctrl);
+    hipLaunchKernelGGL(bin_init_kernel, dim3(1), dim3(1024), 0, s, g, B.totals, B.ctrl);
+    hipLaunchKernelGGL(bin_count_kernel, dim3(g.G), dim3(kBinThreads), hist_lds, s, P, I, g, B.hist, B.totals,
+                       B.emax);
+    hipLaunchKernelGGL(bin_plan_kernel, dim3(1), dim3(1024), 0, s, g, P.N, (const uint32_t*)B.totals,
+                       (const float*)B.emax, B.binoff, B.segs, B.ctrl);
     if (keep)
         hipLaunchKernelGGL(bin_scatter_kernel<true>, dim3(g.G), dim3(kBinThreads), hist_lds, s, P, I, g,
-                           (const uint32_t*)B.hist, B.recs, B.slots, B.info, (const Ctrl*)B.ctrl);
+                           (const uint32_t*)B.hist, (const uint32_t*)B.binoff, (const int4*)B.segs, B.recs, B.slots,
+                           B.info, (const Ctrl*)B.ctrl, out, B.acc64);
     else
         hipLaunchKernelGGL(bin_scatter_kernel<false>, dim3(g.G), dim3(kBinThreads), hist_lds, s, P, I, g,
-                           (const uint32_t*)B.hist, B.recs, (uint32_t*)nullptr, (uint8_t*)nullptr,
-                           (const Ctrl*)B.ctrl);
+                           (const uint32_t*)B.hist, (const uint32_t*)B.binoff, (const int4*)B.segs, B.recs,
+                           (uint32_t*)nullptr, (uint8_t*)nullptr, (const Ctrl*)B.ctrl, out, B.acc64);
 }
 
-template <bool DET>
-void launch_tile_fwd(const Geom& g, const ImageArgs& I, const Buffers& B, float* out, size_t lds, hipStream_t s) {
-    hipLaunchKernelGGL(bin_tile_kernel<DET>, dim3((unsigned)g.max_segments), dim3(kTileThreads), lds, s, g, I,
-                       (const int4*)B.segs, (const uint4*)B.recs, (const Ctrl*)B.ctrl, out, B.acc64);
-}
-
-// zero split tiles -> tiles -> (convert) ; photon-order path on overflow
+// tiles (-> convert); the same launches run the photon-order path on overflow
 void launch_forward_tiles(const PhotonArgs& P, const ImageArgs& I, const Geom& g, const Buffers& B, float* out,
                           hipStream_t s) {
-    const int64_t numel = (int64_t)I.C * I.H * I.W;
     const unsigned nsegs = (unsigned)g.max_segments;
     const size_t tile_lds = (size_t)g.T * g.T * (g.det ? 8 : 4);
-    const unsigned fb = 1024;
     if (g.det) {
-        hipLaunchKernelGGL(bin_zero_split_kernel<unsigned long long>, dim3(nsegs), dim3(kTileThreads), 0, s, g, I,
-                           (const int4*)B.segs, (const Ctrl*)B.ctrl, B.acc64);
-        launch_tile_fwd<true>(g, I, B, out, tile_lds, s);
+        hipLaunchKernelGGL(bin_tile_kernel<true>, dim3(nsegs), dim3(kTileThreads), tile_lds, s, g, P, I,
+                           (const int4*)B.segs, (const uint4*)B.recs, (const Ctrl*)B.ctrl, out, B.acc64);
         hipLaunchKernelGGL(bin_convert_split_kernel, dim3(nsegs), dim3(kTileThreads), 0, s, g, I,
                            (const int4*)B.segs, (const Ctrl*)B.ctrl, (const unsigned long long*)B.acc64, out);
-        hipLaunchKernelGGL(guarded_fill64_kernel, dim3(fb), dim3(256), 0, s, (const Ctrl*)B.ctrl, B.acc64, numel);
-        hipLaunchKernelGGL(guarded_flat_kernel<true>, dim3(fb), dim3(256), 0, s, P, I, (const Ctrl*)B.ctrl, out,
-                           B.acc64);
-        hipLaunchKernelGGL(guarded_convert_kernel, dim3(fb), dim3(256), 0, s, (const Ctrl*)B.ctrl,
-                           (const unsigned long long*)B.acc64, out, numel);
     } else {
-        hipLaunchKernelGGL(bin_zero_split_kernel<float>, dim3(nsegs), dim3(kTileThreads), 0, s, g, I,
-                           (const int4*)B.segs, (const Ctrl*)B.ctrl, out);
-        launch_tile_fwd<false>(g, I, B, out, tile_lds, s);
-        hipLaunchKernelGGL(guarded_fill_kernel, dim3(fb), dim3(256), 0, s, (const Ctrl*)B.ctrl, out, numel);
-        hipLaunchKernelGGL(guarded_flat_kernel<false>, dim3(fb), dim3(256), 0, s, P, I, (const Ctrl*)B.ctrl, out,
-                           (unsigned long long*)nullptr);
+        hipLaunchKernelGGL(bin_tile_kernel<false>, dim3(nsegs), dim3(kTileThreads), tile_lds, s, g, P, I,
+                           (const int4*)B.segs, (const uint4*)B.recs, (const Ctrl*)B.ctrl, out, B.acc64);
     }
 }
 
-// tile adjoint -> finalize ; direct walk of every photon on overflow
+// tile adjoint -> finalize -> direct walk of special photons (of every photon on overflow)
 void launch_backward_tiles(const float* grad, const PhotonArgs& P, const ImageArgs& I, const Geom& g,
                            const Buffers& B, const GradArgs& G, hipStream_t s) {
     const unsigned nsegs = (unsigned)g.max_segments;
     const size_t tile_lds = (size_t)g.T * g.T * 4;
-hipLaunchKernelGGL(bin_tile_bwd_kernel, dim3(nsegs), dim3(kTileThreads), tile_lds, s, g, I, (const int4*)B.segs,
+    hipLaunchKernelGGL(bin_tile_bwd_kernel, dim3(nsegs), dim3(kTileThreads), tile_lds, s, g, I, (const int4*)B.segs,
                        (const uint4*)B.recs, (const Ctrl*)B.ctrl, grad, B.partials);
     int64_t blocks = (P.N + 255) / 256;
     if (blocks > 8192) blocks = 8192;
     if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(bin_finalize_kernel, dim3((unsigned)blocks), dim3(256), 0, s, grad, P, I,
-                       (const Ctrl*)B.ctrl, (const uint8_t*)B.info, (const uint32_t*)B.slots,
-                       (const float4*)B.partials, G);
-    hipLaunchKernelGGL(guarded_bwd_direct_kernel, dim3((unsigned)blocks), dim3(256), 0, s, grad, P, I,
-                       (const Ctrl*)B.ctrl, G);
+    hipLaunchKernelGGL(bin_finalize_kernel, dim3((unsigned)blocks), dim3(256), 0, s, P, (const Ctrl*)B.ctrl,
+                       (const uint8_t*)B.info, (const uint32_t*)B.slots, (const float4*)B.partials, G);
+    hipLaunchKernelGGL(bin_direct_kernel, dim3((unsigned)blocks), dim3(256), 0, s, grad, P, I, (const Ctrl*)B.ctrl,
+                       (const uint8_t*)B.info, G);
 }
 
 // Photon-order forward for problems the binned path cannot take.
@@ -1062,7 +1107,7 @@ int binned_forward(const PhotonArgs& P, const ImageArgs& I, int32_t mode, const 
         return PDS_ERR_WORKSPACE;
     }
     const Buffers B = buffers(ws, L);
-    launch_build(P, I, g, B, keep, s);
+    launch_build(P, I, g, B, keep, out, s);
     launch_forward_tiles(P, I, g, B, out, s);
     return hipGetLastError() == hipSuccess ? PDS_OK : PDS_ERR_HIP;
 }
@@ -1080,7 +1125,8 @@ int binned_backward(const float* grad, const PhotonArgs& P, const ImageArgs& I, 
         return PDS_ERR_WORKSPACE;
     }
     const Buffers B = buffers(ws, L);
-    if (!reuse) launch_build(P, I, g, B, /*keep=*/true, s);
+    // no image output: the scatter clears nothing
+    if (!reuse) launch_build(P, I, g, B, /*keep=*/true, nullptr, s);
     launch_backward_tiles(grad, P, I, g, B, G, s);
     return hipGetLastError() == hipSuccess ? PDS_OK : PDS_ERR_HIP;
 }
