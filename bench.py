@@ -44,7 +44,7 @@ def make_workload(name):
         desc = "plain: 2^18 photons -> 1x256x256 fp32, cap 8, uniform"
     elif name == "batched":
         p = WL.batched()
-        desc = "batched: 16 ragged splats (logU(2^18,2^22) photons) -> 16x512x512 fp32, cap 8"
+        desc = "batched: 16 ragged splats (logU(2^18,2^22) photons, 19365832 total) -> 16x512x512 fp32, cap 8"
     else:
         raise SystemExit(f"unknown workload {name}")
     return p, desc
@@ -90,7 +90,15 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain-atomic comparator")
     ap.add_argument("--stateless", action="store_true",
                     help="time pds_forward + pds_backward (the backward bins again) instead of the PDS pairing")
+    ap.add_argument("--graph", action="store_true",
+                    help="capture the step in a HIP graph and time its replays (default for --workload batched)")
+    ap.add_argument("--eager", action="store_true", help="never use a HIP graph")
+    ap.add_argument("--tuned", action="store_true", help="use the autotuner's cached tuning for this problem")
     args = ap.parse_args()
+    if args.workload == "batched" and not args.eager:
+        args.graph = True
+    if args.eager:
+        args.graph = False
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (photon sharding plus one image "
               "all-reduce is the design, not built); run with --gpus 1", file=sys.stderr)
@@ -107,6 +115,7 @@ def main():
     torch.cuda.set_device(dev)
     log(f"[bench] generating workload {args.workload}")
     p, desc = make_workload(args.workload)
+    offsets = np.concatenate([[0], np.cumsum(np.bincount(p.cp[0], minlength=p.size[0]))])
     grad_np = WL.upstream_grad(p.size, seed=2)
     E, xp, M, cp, radius = p.to_torch(dev)
     grad = torch.from_numpy(grad_np).to(dev)
@@ -120,37 +129,101 @@ def main():
         keep = not args.stateless
         if ev is not None:
             ev[0].record()
-        img, bins = _forward(E, xp, M, cp, radius, p.size, p.cap, mode, None, keep_bins=keep)
+        img, bins = _forward(E, xp, M, cp, radius, p.size, p.cap, mode, tuning, keep_bins=keep)
         if ev is not None:
             ev[1].record()
-        g = _backward(grad, E, xp, M, cp, radius, p.cap, mode, None, bins=bins)
+        g = _backward(grad, E, xp, M, cp, radius, p.cap, mode, tuning, bins=bins)
         if ev is not None:
             ev[2].record()
         return img, g
 
-    def timed(mode, steps, warmup):
+    def timed(run, steps, warmup, split=True):
+        """run(ev) does one step, recording ev[0..2] (start, after forward, end)."""
         for _ in range(warmup):
-            step(mode)
+            run(None)
         evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for k in range(steps):
-            step(mode, evs[k])
+            run(evs[k])
         torch.cuda.synchronize()
         wall = (time.perf_counter() - t0) / steps
+        if not split:
+            tot = sorted(e[0].elapsed_time(e[2]) for e in evs)
+            return wall, tot[len(tot) // 2] * 1e-3, None
         fwd = sorted(e[0].elapsed_time(e[1]) for e in evs)
         bwd = sorted(e[1].elapsed_time(e[2]) for e in evs)
         return wall, fwd[len(fwd) // 2] * 1e-3, bwd[len(bwd) // 2] * 1e-3
 
-    log(f"[bench] {desc}; mode={args.mode}; warmup {args.warmup}, steps {args.steps}")
-    wall, t_fwd, t_bwd = timed(args.mode, args.steps, args.warmup)
-    log(f"[bench] {args.mode}: {wall * 1e3:.3f} ms/step (fwd {t_fwd * 1e6:.1f} us, bwd {t_bwd * 1e6:.1f} us)")
+    tuning = "cached" if args.tuned else None
+    graph = None
+    if args.graph:
+        # the whole step (forward keeping its bins + backward) captured once
+        # and replayed: HIP graph launch instead of ~10 kernel launches
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            for _ in range(2):
+                step(args.mode)
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            step(args.mode)
+
+    def run_main(ev):
+        if graph is None:
+            step(args.mode, ev)
+            return
+        if ev is not None:
+            ev[0].record()
+        graph.replay()
+        if ev is not None:
+            ev[1].record()
+            ev[2].record()
+
+    log(f"[bench] {desc}; mode={args.mode}{' (HIP graph)' if graph is not None else ''}; "
+        f"warmup {args.warmup}, steps {args.steps}")
+    if graph is None:
+        wall, t_fwd, t_bwd = timed(run_main, args.steps, args.warmup)
+        log(f"[bench] {args.mode}: {wall * 1e3:.3f} ms/step (fwd {t_fwd * 1e6:.1f} us, bwd {t_bwd * 1e6:.1f} us)")
+    else:
+        wall, t_step, _ = timed(run_main, args.steps, args.warmup, split=False)
+        # forward and backward split measured on the eager sequence
+        _, t_fwd, t_bwd = timed(lambda ev: step(args.mode, ev), max(3, args.steps // 4), 1)
+        log(f"[bench] {args.mode} graph: {wall * 1e3:.3f} ms/step (graph {t_step * 1e6:.1f} us; eager fwd "
+            f"{t_fwd * 1e6:.1f} us, bwd {t_bwd * 1e6:.1f} us)")
 
     plain = None
     if not args.no_plain and args.mode != "plain":
-        pw, pf, pb = timed("plain", max(3, args.steps // 4), 2)
-        plain = {"value": N / pw, "ms_per_step": pw * 1e3, "fwd_us": pf * 1e6, "bwd_us": pb * 1e6}
-        log(f"[bench] plain-atomic comparator: {pw * 1e3:.3f} ms/step")
+        if args.workload == "batched":
+            # the reference's way: one splat after another, each into its own image
+            parts = []
+            for k in range(p.size[0]):
+                a, b = int(offsets[k]), int(offsets[k + 1])
+                parts.append((E[a:b].contiguous(), xp[:, a:b].contiguous(), M[:, :, a:b].contiguous(),
+                              torch.zeros((1, b - a), dtype=torch.int64, device=dev), radius[a:b].contiguous(),
+                              grad[k:k + 1].contiguous()))
+
+            def run_plain(ev):
+                if ev is not None:
+                    ev[0].record()
+                outs = [_forward(e, x, m, c, r, (1, H, W), p.cap, "plain", None, keep_bins=False)[0]
+                        for e, x, m, c, r, _ in parts]
+                if ev is not None:
+                    ev[1].record()
+                for e, x, m, c, r, g in parts:
+                    _backward(g, e, x, m, c, r, p.cap, "plain", None)
+                if ev is not None:
+                    ev[2].record()
+                return outs
+            pw, pf, pb = timed(run_plain, max(3, args.steps // 4), 1)
+            label = f"{p.size[0]} sequential plain-atomic launches"
+        else:
+            pw, pf, pb = timed(lambda ev: step("plain", ev), max(3, args.steps // 4), 2)
+            label = "plain-atomic comparator"
+        plain = {"value": N / pw, "ms_per_step": pw * 1e3, "fwd_us": pf * 1e6, "bwd_us": pb * 1e6, "what": label}
+        log(f"[bench] {label}: {pw * 1e3:.3f} ms/step")
 
     # dominant pass: the backward is one kernel launch; the forward's kernels
     # are bracketed together (zero fill + splat, or binning + tile passes)
@@ -187,7 +260,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded numpy PCG64 photons and N(0,1) upstream gradient)",
         "config": {"workload": desc, "photons": N, "image": [C, H, W], "max_pixel_radius": p.cap,
-                   "mode": args.mode, "parallelism": "single GPU"},
+                   "mode": args.mode, "hip_graph": graph is not None, "tuning": tuning or "default",
+                   "parallelism": "single GPU"},
         "fwd_us": round(t_fwd * 1e6, 2),
         "bwd_us": round(t_bwd * 1e6, 2),
         "roofline": roofline,

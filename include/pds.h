@@ -111,6 +111,27 @@ int pds_forward_bins_f32(const float* Ep, const float* xp, const float* Mp, cons
                          int32_t max_pixel_radius, int32_t mode, const pds_tuning* t, float* out,
                          void* bins, size_t bins_bytes, void* stream);
 
+/* What a binning in ``bins`` holds, for tuning: filled by pds_bins_stats()
+ * after pds_forward_bins_f32 on the same arguments.  No counterpart in the
+ * reference. */
+typedef struct pds_bins_stats {
+    int64_t records;      /* (photon, tile) records written */
+    int64_t capacity;     /* records the workspace holds */
+    int32_t segments;     /* tile segments (one workgroup each) */
+    int32_t overflow;     /* 1: records exceeded capacity, the photon-order path ran */
+    int32_t tile;         /* tile edge used */
+    int32_t seg_photons;  /* records per segment used */
+    int32_t bins;         /* channel x tile bins */
+    int32_t block_photons;/* photons per binning workgroup used */
+    int32_t reserved[2];
+} pds_bins_stats;
+
+/* Read the statistics of ``bins``; synchronises ``stream`` (diagnostic, not
+ * for the hot loop). */
+int pds_bins_stats_get(const void* bins, size_t bins_bytes, int64_t N, int32_t C, int32_t H, int32_t W,
+                       int32_t max_pixel_radius, int32_t mode, const pds_tuning* t, pds_bins_stats* out,
+                       void* stream);
+
 /* pds_backward_f32 reusing ``bins`` from pds_forward_bins_f32 on the same inputs. */
 int pds_backward_bins_f32(const float* grad, const float* Ep, const float* xp, const float* Mp,
                           const int64_t* cp, const float* radius, int64_t N, int32_t C, int32_t H,

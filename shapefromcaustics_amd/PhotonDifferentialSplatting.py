@@ -97,6 +97,17 @@ def _tun_ref(tun):
     return ctypes.byref(tun) if tun is not None else None
 
 
+def _resolve_tuning(tuning, N, size, cap, mode):
+    """``tuning``: None (built-in defaults), a dict / PdsTuning, or ``"cached"``
+    (the autotuner's cached choice for this problem class, defaults if none)."""
+    if isinstance(tuning, str):
+        if tuning != "cached":
+            raise ValueError(f"unknown tuning {tuning!r}; expected None, a dict or 'cached'")
+        from . import autotune
+        tuning = autotune.lookup(N, size, cap, mode)
+    return _native.tuning_struct(tuning)
+
+
 def _forward(Ep, xp, Mp, cp, radius, output_size, max_pixel_radius, mode, tuning, keep_bins):
     _check_photons(Ep, xp, Mp, cp, radius)
     size = [int(s) for s in output_size]
@@ -105,7 +116,7 @@ def _forward(Ep, xp, Mp, cp, radius, output_size, max_pixel_radius, mode, tuning
     C, H, W = size
     N = Ep.size(0)
     m = _mode(mode)
-    tun = _native.tuning_struct(tuning)
+    tun = _resolve_tuning(tuning, N, size, max_pixel_radius, mode)
     lib = _native.lib()
     cap = int(max_pixel_radius)
     with torch.cuda.device(Ep.device):
@@ -138,7 +149,7 @@ def _backward(grad_pds, Ep, xp, Mp, cp, radius, max_pixel_radius, mode, tuning, 
     C, H, W = grad_pds.shape
     N = Ep.size(0)
     m = _mode(mode)
-    tun = _native.tuning_struct(tuning)
+    tun = _resolve_tuning(tuning, N, (C, H, W), max_pixel_radius, mode)
     lib = _native.lib()
     cap = int(max_pixel_radius)
     with torch.cuda.device(Ep.device):
@@ -155,6 +166,18 @@ def _backward(grad_pds, Ep, xp, Mp, cp, radius, max_pixel_radius, mode, tuning, 
                 _tun_ref(tun), _ptr(gE), _ptr(gx), _ptr(gM), _ptr(ws), nbytes, ctypes.c_void_p(stream))
     _native.check(rc, "pds_backward")
     return [gE, gx, gM]
+
+
+def _bins_stats(bins, N, output_size, max_pixel_radius, mode="auto", tuning=None) -> dict:
+    """Statistics of a binning kept by ``_forward(keep_bins=True)`` (synchronises)."""
+    C, H, W = [int(s) for s in output_size]
+    tun = _resolve_tuning(tuning, N, (C, H, W), max_pixel_radius, mode)
+    st = _native.PdsBinsStats()
+    rc = _native.lib().pds_bins_stats_get(_ptr(bins), bins.numel(), N, C, H, W, int(max_pixel_radius), _mode(mode),
+                                          _tun_ref(tun), ctypes.byref(st),
+                                          ctypes.c_void_p(torch.cuda.current_stream(bins.device).cuda_stream))
+    _native.check(rc, "pds_bins_stats_get")
+    return {k: getattr(st, k) for k, _ in st._fields_ if k != "reserved"}
 
 
 def pds_forward(Ep: torch.Tensor, xp: torch.Tensor, Mp: torch.Tensor, cp: torch.Tensor, radius: torch.Tensor,

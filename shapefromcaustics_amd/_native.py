@@ -30,6 +30,14 @@ class PdsTuning(ctypes.Structure):
                 ("capacity_pct", ctypes.c_int32), ("reserved", ctypes.c_int32 * 4)]
 
 
+class PdsBinsStats(ctypes.Structure):
+    """Mirror of ``pds_bins_stats`` (include/pds.h)."""
+
+    _fields_ = [("records", ctypes.c_int64), ("capacity", ctypes.c_int64), ("segments", ctypes.c_int32),
+                ("overflow", ctypes.c_int32), ("tile", ctypes.c_int32), ("seg_photons", ctypes.c_int32),
+                ("bins", ctypes.c_int32), ("block_photons", ctypes.c_int32), ("reserved", ctypes.c_int32 * 2)]
+
+
 _lib = None
 
 _vp = ctypes.c_void_p
@@ -49,6 +57,8 @@ SIGNATURES = {
                              ctypes.c_int),
     "pds_backward_bins_f32": ([_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _i32, _i32, _i32, _tp, _vp, _vp, _vp,
                                _vp, _sz, _vp], ctypes.c_int),
+    "pds_bins_stats_get": ([_vp, _sz, _i64, _i32, _i32, _i32, _i32, _i32, _tp, ctypes.POINTER(PdsBinsStats), _vp],
+                           ctypes.c_int),
     "pds_last_error": ([], ctypes.c_char_p),
     "pds_version": ([], ctypes.c_char_p),
 }

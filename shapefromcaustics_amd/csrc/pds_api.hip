@@ -222,4 +222,18 @@ int pds_backward_bins_f32(const float* grad, const float* Ep, const float* xp, c
                          grad_Mp, bins, bins_bytes, true, stream);
 }
 
+int pds_bins_stats_get(const void* bins, size_t bins_bytes, int64_t N, int32_t C, int32_t H, int32_t W,
+                       int32_t max_pixel_radius, int32_t mode, const pds_tuning* t, pds_bins_stats* out,
+                       void* stream) {
+    g_last_error[0] = '\0';
+    if (!out) return fail(PDS_ERR_ARGUMENT, "null stats pointer");
+    if (N < 0 || C <= 0 || H <= 0 || W <= 0) return fail(PDS_ERR_ARGUMENT, "invalid problem size");
+    if (mode < PDS_MODE_AUTO || mode > PDS_MODE_DETERMINISTIC) return fail(PDS_ERR_ARGUMENT, "unknown mode %d", mode);
+    const int rc = pds::binned_stats(bins, bins_bytes, N, image_args(C, H, W, max_pixel_radius), mode, t, out,
+                                     (hipStream_t)stream);
+    if (rc == PDS_ERR_HIP) return hip_fail(hipGetLastError(), "bins stats");
+    if (rc) return fail(rc, "bins stats: %s", pds::binned_last_error());
+    return PDS_OK;
+}
+
 }  // extern "C"

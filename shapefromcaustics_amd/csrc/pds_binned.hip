@@ -1131,6 +1131,34 @@ int binned_backward(const float* grad, const PhotonArgs& P, const ImageArgs& I, 
     return hipGetLastError() == hipSuccess ? PDS_OK : PDS_ERR_HIP;
 }
 
+int binned_stats(const void* ws, size_t ws_bytes, int64_t N, const ImageArgs& I, int32_t mode, const pds_tuning* t,
+                 pds_bins_stats* out, hipStream_t s) {
+    Geom g;
+    if (!binnable(I, N, mode, t, g)) {
+        g_err = "this problem has no bins";
+        return PDS_ERR_ARGUMENT;
+    }
+    const Layout L = make_layout(g, N, I.C, I.H, I.W, /*keep=*/true);
+    if (ws == nullptr || ws_bytes < L.end) {
+        g_err = "bins buffer too small";
+        return PDS_ERR_WORKSPACE;
+    }
+    Ctrl c;
+    if (hipMemcpyAsync(&c, (const unsigned char*)ws + L.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return PDS_ERR_HIP;
+    out->records = (int64_t)c.total;
+    out->capacity = g.cap_records;
+    out->segments = (int32_t)c.nseg;
+    out->overflow = (int32_t)c.overflow;
+    out->tile = g.T;
+    out->seg_photons = g.seg;
+    out->bins = g.nbins;
+    out->block_photons = (int32_t)g.per_block;
+    out->reserved[0] = out->reserved[1] = 0;
+    return PDS_OK;
+}
+
 bool binned_possible(const ImageArgs& I, int64_t N, int32_t mode, const pds_tuning* t) {
     Geom g;
     return binnable(I, N, mode, t, g);

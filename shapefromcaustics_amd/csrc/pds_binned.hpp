@@ -20,6 +20,9 @@ int binned_forward(const PhotonArgs& P, const ImageArgs& I, int32_t mode, const 
 // Backward through the bins; reuse: ws holds the binning of a keep-forward of the same inputs.
 int binned_backward(const float* grad, const PhotonArgs& P, const ImageArgs& I, int32_t mode, const pds_tuning* t,
                     const GradArgs& G, void* ws, size_t ws_bytes, bool reuse, hipStream_t s);
+// Statistics of a kept binning (synchronises s).
+int binned_stats(const void* ws, size_t ws_bytes, int64_t N, const ImageArgs& I, int32_t mode, const pds_tuning* t,
+                 pds_bins_stats* out, hipStream_t s);
 bool binned_possible(const ImageArgs& I, int64_t N, int32_t mode, const pds_tuning* t);
 const char* binned_last_error();
 
