@@ -1,0 +1,356 @@
+// hf_device.hpp -- device geometry of the height-field caustic pipeline:
+// the substrate scene, a grid-traversal ray tracer with the reference's
+// trace contract, shading, differential transfer and refraction.
+//
+// Scene (model/renderable_object.py:81-193, :249-308): object 0 is the glass
+// substrate over [-1, 1]^2 -- its top is the height field h (R x R vertices,
+// two triangles per cell), its bottom the plane z = 0, and four side walls
+// join them; object 1 is the ground plane z = sensor_z over [-1, 1]^2 (two
+// triangles).  Triangle order inside object 0: top 2(R-1)^2, bottom
+// 2(R-1)^2, front (y = -1), back (y = +1), left (x = -1), right (x = +1),
+// 2(R-1) each -- the order update_from_height_field builds, so primitive
+// indices match the reference's.
+//
+// Trace contract (PyOptix/RayTrace.cpp:233-291, kernel/ray_programs.cu:84-101):
+// closest hit with t >= 1e-3 in units of the (possibly non-unit) direction,
+// barycentrics (b1, b2) with hit = (1 - b1 - b2) v0 + b1 v1 + b2 v2, object
+// and primitive index; a miss gives t = -2, b = (-1, -1), indices -1.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hf {
+
+constexpr float kTMin = 1e-3f;
+constexpr float kMissT = -2.0f;
+constexpr int kMaxBounces = 8;
+
+struct V3 {
+    float x, y, z;
+};
+
+__device__ __forceinline__ V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+__device__ __forceinline__ V3 operator+(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ V3 operator-(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ V3 operator-(V3 a) { return v3(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ V3 operator*(float s, V3 a) { return v3(s * a.x, s * a.y, s * a.z); }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return fmaf(a.x, b.x, fmaf(a.y, b.y, a.z * b.z)); }
+__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+    return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float norm(V3 a) { return sqrtf(dot(a, a)); }
+// model/utils.py:25-31: v / |v| for |v| > 0, else 0
+__device__ __forceinline__ V3 normalize(V3 a) {
+    const float n = norm(a);
+    return n > 0.0f ? (1.0f / n) * a : v3(0.f, 0.f, 0.f);
+}
+
+struct SceneArgs {
+    const float* h;        // [R, R] heights, row-major (row = y, column = x)
+    const float* normals;  // [R, R, 3] unit Sobel normals of the top surface
+    int32_t R;
+    float delta;           // 2 / (R - 1)
+    float inv_delta;
+    float sensor_z;        // ground plane height
+    float hmax;            // max(h) (top-surface traversal bound)
+};
+
+__device__ __forceinline__ float height(const SceneArgs& S, int r, int c) { return S.h[r * S.R + c]; }
+__device__ __forceinline__ float coord(const SceneArgs& S, int k) { return -1.0f + (float)k * S.delta; }
+__device__ __forceinline__ V3 top_pt(const SceneArgs& S, int r, int c) {
+    return v3(coord(S, c), coord(S, r), height(S, r, c));
+}
+__device__ __forceinline__ V3 bot_pt(const SceneArgs& S, int r, int c) { return v3(coord(S, c), coord(S, r), 0.0f); }
+
+// Primitive index bases inside object 0.
+struct PrimBase {
+    int top, bottom, front, back, left, right;
+};
+__device__ __forceinline__ PrimBase prim_base(int R) {
+    const int cells = (R - 1) * (R - 1), e = R - 1;
+    PrimBase b;
+    b.top = 0;
+    b.bottom = 2 * cells;
+    b.front = 4 * cells;
+    b.back = b.front + 2 * e;
+    b.left = b.back + 2 * e;
+    b.right = b.left + 2 * e;
+    return b;
+}
+
+// Where a vertex of a triangle comes from: a top-surface grid vertex (its
+// height and normal depend on h) or a constant.
+struct TriVerts {
+    V3 v[3];
+    V3 n[3];        // vertex normals
+    int vr[3], vc[3];  // grid vertex of a top vertex, vr = -1 for constant vertices
+};
+
+// Vertices and vertex normals of (obj, prim) -- grid_shuffle order
+// (model/renderable_object.py:52-78).
+__device__ __forceinline__ TriVerts tri_verts(const SceneArgs& S, int obj, int prim) {
+    TriVerts T;
+    const int R = S.R, e = R - 1;
+    for (int k = 0; k < 3; ++k) T.vr[k] = -1, T.vc[k] = 0;
+    if (obj == 1) {  // ground plane
+        const float z = S.sensor_z;
+        if (prim == 0) {
+            T.v[0] = v3(-1.f, -1.f, z), T.v[1] = v3(1.f, -1.f, z), T.v[2] = v3(-1.f, 1.f, z);
+        } else {
+            T.v[0] = v3(-1.f, 1.f, z), T.v[1] = v3(1.f, -1.f, z), T.v[2] = v3(1.f, 1.f, z);
+        }
+        for (int k = 0; k < 3; ++k) T.n[k] = v3(0.f, 0.f, 1.f);
+        return T;
+    }
+    const PrimBase B = prim_base(R);
+    auto top = [&](int k, int r, int c) {
+        T.v[k] = top_pt(S, r, c);
+        const float* nn = S.normals + 3 * (r * R + c);
+        T.n[k] = v3(nn[0], nn[1], nn[2]);
+        T.vr[k] = r;
+        T.vc[k] = c;
+    };
+    auto bot = [&](int k, int r, int c, V3 n) {
+        T.v[k] = bot_pt(S, r, c);
+        T.n[k] = n;
+    };
+    auto wall = [&](int k, int r, int c, V3 n) {  // top vertex of a wall: height from h, constant normal
+        T.v[k] = top_pt(S, r, c);
+        T.n[k] = n;
+        T.vr[k] = r;
+        T.vc[k] = c;
+    };
+    if (prim < B.bottom) {
+        const int cell = prim >> 1, r = cell / e, c = cell - r * e;
+        if ((prim & 1) == 0) {
+            top(0, r, c), top(1, r, c + 1), top(2, r + 1, c);
+        } else {
+            top(0, r, c + 1), top(1, r + 1, c + 1), top(2, r + 1, c);
+        }
+    } else if (prim < B.front) {
+        const V3 n = v3(0.f, 0.f, -1.f);
+        const int q = prim - B.bottom, cell = q >> 1, r = cell / e, c = cell - r * e;
+        if ((q & 1) == 0) {
+            bot(0, r, c, n), bot(1, r + 1, c, n), bot(2, r, c + 1, n);
+        } else {
+            bot(0, r, c + 1, n), bot(1, r + 1, c, n), bot(2, r + 1, c + 1, n);
+        }
+    } else if (prim < B.back) {  // front, y = -1: grid rows (bottom, top) of row 0
+        const V3 n = v3(0.f, -1.f, 0.f);
+        const int q = prim - B.front, c = q >> 1;
+        if ((q & 1) == 0) {
+            bot(0, 0, c, n), bot(1, 0, c + 1, n), wall(2, 0, c, n);
+        } else {
+            bot(0, 0, c + 1, n), wall(1, 0, c + 1, n), wall(2, 0, c, n);
+        }
+    } else if (prim < B.left) {  // back, y = +1, flipped
+        const V3 n = v3(0.f, 1.f, 0.f);
+        const int q = prim - B.back, c = q >> 1;
+        if ((q & 1) == 0) {
+            bot(0, e, c, n), wall(1, e, c, n), bot(2, e, c + 1, n);
+        } else {
+            bot(0, e, c + 1, n), wall(1, e, c, n), wall(2, e, c + 1, n);
+        }
+    } else if (prim < B.right) {  // left, x = -1: grid columns (bottom, top) of column 0
+        const V3 n = v3(-1.f, 0.f, 0.f);
+        const int q = prim - B.left, r = q >> 1;
+        if ((q & 1) == 0) {
+            bot(0, r, 0, n), wall(1, r, 0, n), bot(2, r + 1, 0, n);
+        } else {
+            wall(0, r, 0, n), wall(1, r + 1, 0, n), bot(2, r + 1, 0, n);
+        }
+    } else {  // right, x = +1, flipped
+        const V3 n = v3(1.f, 0.f, 0.f);
+        const int q = prim - B.right, r = q >> 1;
+        if ((q & 1) == 0) {
+            bot(0, r, e, n), bot(1, r + 1, e, n), wall(2, r, e, n);
+        } else {
+            wall(0, r, e, n), bot(1, r + 1, e, n), wall(2, r + 1, e, n);
+        }
+    }
+    return T;
+}
+
+struct Hit {
+    float t, b1, b2;
+    int obj, prim;
+};
+
+__device__ __forceinline__ Hit miss_hit() { return Hit{kMissT, -1.0f, -1.0f, -1, -1}; }
+
+// Moeller-Trumbore; keeps the closer hit with t >= kTMin (no culling).
+__device__ __forceinline__ void mt_test(V3 o, V3 d, V3 v0, V3 v1, V3 v2, int obj, int prim, Hit& best) {
+    const V3 e1 = v1 - v0, e2 = v2 - v0;
+    const V3 pv = cross(d, e2);
+    const float det = dot(e1, pv);
+    if (det == 0.0f) return;
+    const float inv = 1.0f / det;
+    const V3 tv = o - v0;
+    const float u = dot(tv, pv) * inv;
+    if (!(u >= 0.0f)) return;
+    const V3 qv = cross(tv, e1);
+    const float v = dot(d, qv) * inv;
+    if (!(v >= 0.0f) || u + v > 1.0f) return;
+    const float t = dot(e2, qv) * inv;
+    if (!(t >= kTMin)) return;
+    if (best.obj < 0 || t < best.t) best = Hit{t, u, v, obj, prim};
+}
+
+__device__ __forceinline__ void test_prim(const SceneArgs& S, V3 o, V3 d, int obj, int prim, Hit& best) {
+    const TriVerts T = tri_verts(S, obj, prim);
+    mt_test(o, d, T.v[0], T.v[1], T.v[2], obj, prim, best);
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Closest hit of the ray (o, d) with the scene.
+__device__ inline Hit trace(const SceneArgs& S, V3 o, V3 d) {
+    Hit best = miss_hit();
+    const int R = S.R, e = R - 1;
+    const PrimBase B = prim_base(R);
+    // ground plane (2 triangles)
+    mt_test(o, d, v3(-1.f, -1.f, S.sensor_z), v3(1.f, -1.f, S.sensor_z), v3(-1.f, 1.f, S.sensor_z), 1, 0, best);
+    mt_test(o, d, v3(-1.f, 1.f, S.sensor_z), v3(1.f, -1.f, S.sensor_z), v3(1.f, 1.f, S.sensor_z), 1, 1, best);
+    // bottom plane z = 0: the two triangles of the cell under the plane hit
+    if (d.z != 0.0f) {
+        const float tp = -o.z / d.z;
+        if (tp >= 0.0f) {
+            const float x = fmaf(tp, d.x, o.x), y = fmaf(tp, d.y, o.y);
+            if (x >= -1.0f && x <= 1.0f && y >= -1.0f && y <= 1.0f) {
+                const int c = clampi((int)floorf((x + 1.0f) * S.inv_delta), 0, e - 1);
+                const int r = clampi((int)floorf((y + 1.0f) * S.inv_delta), 0, e - 1);
+                // neighbours too: the plane hit may sit on a cell edge in float
+                for (int dr = -1; dr <= 1; ++dr)
+                    for (int dc = -1; dc <= 1; ++dc) {
+                        const int rr = r + dr, cc = c + dc;
+                        if (rr < 0 || rr >= e || cc < 0 || cc >= e) continue;
+                        const int base = B.bottom + 2 * (rr * e + cc);
+                        test_prim(S, o, d, 0, base, best);
+                        test_prim(S, o, d, 0, base + 1, best);
+                    }
+            }
+        }
+    }
+    // side walls x = +-1, y = +-1: the two triangles of the segment at the plane hit
+    if (d.x != 0.0f) {
+        for (int side = 0; side < 2; ++side) {
+            const float X = side ? 1.0f : -1.0f;
+            const float tp = (X - o.x) / d.x;
+            if (tp >= 0.0f) {
+                const float y = fmaf(tp, d.y, o.y);
+                const int r = clampi((int)floorf((y + 1.0f) * S.inv_delta), 0, e - 1);
+                const int base = side ? B.right : B.left;
+                for (int dr = -1; dr <= 1; ++dr) {
+                    const int rr = r + dr;
+                    if (rr < 0 || rr >= e) continue;
+                    test_prim(S, o, d, 0, base + 2 * rr, best);
+                    test_prim(S, o, d, 0, base + 2 * rr + 1, best);
+                }
+            }
+        }
+    }
+    if (d.y != 0.0f) {
+        for (int side = 0; side < 2; ++side) {
+            const float Y = side ? 1.0f : -1.0f;
+            const float tp = (Y - o.y) / d.y;
+            if (tp >= 0.0f) {
+                const float x = fmaf(tp, d.x, o.x);
+                const int c = clampi((int)floorf((x + 1.0f) * S.inv_delta), 0, e - 1);
+                const int base = side ? B.back : B.front;
+                for (int dc = -1; dc <= 1; ++dc) {
+                    const int cc = c + dc;
+                    if (cc < 0 || cc >= e) continue;
+                    test_prim(S, o, d, 0, base + 2 * cc, best);
+                    test_prim(S, o, d, 0, base + 2 * cc + 1, best);
+                }
+            }
+        }
+    }
+    // top surface: walk the cells under the ray's xy path while its z lies in
+    // [0, hmax] (the surface's range); the first cell with a hit holds the
+    // closest top hit.  Cells are visited in increasing t.
+    {
+        float t0 = 0.0f, t1 = INFINITY;
+        // clip to the square [-1, 1]^2
+        auto clip = [&](float oo, float dd) {
+            if (dd == 0.0f) {
+                if (oo < -1.0f || oo > 1.0f) t1 = -1.0f;
+                return;
+            }
+            float a = (-1.0f - oo) / dd, b = (1.0f - oo) / dd;
+            if (a > b) {
+                const float tmp = a;
+                a = b;
+                b = tmp;
+            }
+            t0 = fmaxf(t0, a);
+            t1 = fminf(t1, b);
+        };
+        clip(o.x, d.x);
+        clip(o.y, d.y);
+        // clip to the slab 0 <= z <= hmax
+        if (d.z == 0.0f) {
+            if (o.z < 0.0f || o.z > S.hmax) t1 = -1.0f;
+        } else {
+            float a = (0.0f - o.z) / d.z, b = (S.hmax - o.z) / d.z;
+            if (a > b) {
+                const float tmp = a;
+                a = b;
+                b = tmp;
+            }
+            t0 = fmaxf(t0, a);
+            t1 = fminf(t1, b);
+        }
+        if (t0 <= t1 && t1 >= 0.0f) {
+            // start cell (slightly inside the entry point) and DDA steps
+            const float tm = t0;
+            const float x = fmaf(tm, d.x, o.x), y = fmaf(tm, d.y, o.y);
+            int c = clampi((int)floorf((x + 1.0f) * S.inv_delta), 0, e - 1);
+            int r = clampi((int)floorf((y + 1.0f) * S.inv_delta), 0, e - 1);
+            const int sc = d.x > 0.0f ? 1 : (d.x < 0.0f ? -1 : 0);
+            const int sr = d.y > 0.0f ? 1 : (d.y < 0.0f ? -1 : 0);
+            const float tdx = sc ? S.delta / fabsf(d.x) : INFINITY;
+            const float tdy = sr ? S.delta / fabsf(d.y) : INFINITY;
+            float tnx = sc ? ((coord(S, c + (sc > 0 ? 1 : 0)) - o.x) / d.x) : INFINITY;
+            float tny = sr ? ((coord(S, r + (sr > 0 ? 1 : 0)) - o.y) / d.y) : INFINITY;
+            Hit cell_best = miss_hit();
+            for (int guard = 0; guard < 4 * R + 8; ++guard) {
+                // this cell and, against float edge cases, its previous neighbours
+                const int base = B.top + 2 * (r * e + c);
+                test_prim(S, o, d, 0, base, cell_best);
+                test_prim(S, o, d, 0, base + 1, cell_best);
+                const float texit = fminf(tnx, tny);
+                if (cell_best.obj >= 0 && cell_best.t <= texit * (1.0f + 1e-6f) + 1e-6f) break;
+                if (texit > t1) break;
+                if (tnx < tny) {
+                    c += sc;
+                    tnx += tdx;
+                } else {
+                    r += sr;
+                    tny += tdy;
+                }
+                if (c < 0 || c >= e || r < 0 || r >= e) break;
+            }
+            if (cell_best.obj >= 0 && (best.obj < 0 || cell_best.t < best.t)) best = cell_best;
+        }
+    }
+    return best;
+}
+
+// ---------------------------------------------------------------------------
+// shading and optics
+// ---------------------------------------------------------------------------
+
+// model/renderable_object.py:383-389 (cw = False)
+__device__ __forceinline__ V3 geometric_normal(const TriVerts& T) {
+    return normalize(cross(T.v[1] - T.v[0], T.v[2] - T.v[0]));
+}
+
+// model/utils.py:65-74 + renormalisation (model/renderable_object.py:505-506)
+__device__ __forceinline__ V3 shading_raw(const TriVerts& T, float b1, float b2) {
+    const float w0 = 1.0f - b1 - b2;
+    return w0 * T.n[0] + b1 * T.n[1] + b2 * T.n[2];
+}
+
+}  // namespace hf

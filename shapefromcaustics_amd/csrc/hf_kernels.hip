@@ -1,0 +1,781 @@
+// hf_kernels.hip -- height-field caustic kernels (include/hf.h): Sobel
+// normals, the trace contract, the fused photon path (emit -> trace ->
+// transfer -> refract -> ... -> ground -> splat inputs), its reverse-mode
+// adjoint with recomputation, and the Landweber data term and update.
+//
+// One thread per photon.  The path kernels keep a photon's whole path in
+// registers; the adjoint re-runs it, keeping a per-bounce tape (20 floats a
+// bounce, at most 8 bounces) in private memory, and sweeps it backwards.
+// Per-vertex gradients (Sobel normals, vertex heights) are accumulated with
+// device float atomics into [R, R] arrays, then one gather kernel applies the
+// transposed Sobel/normalisation to give dL/dh.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+
+#include "hf.h"
+#include "hf_device.hpp"
+#include "pds.h"
+
+namespace hf {
+
+namespace {
+
+thread_local char g_err[256] = "";
+
+int fail(int code, const char* msg) {
+    snprintf(g_err, sizeof(g_err), "%s", msg);
+    return code;
+}
+
+inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// workspace: [0] hmax (float) ; then grad_n [R,R,3], grad_z [R,R], grad_g [R,R,2]
+struct WsLayout {
+    size_t hmax, grad_n, grad_z, grad_g, end;
+};
+WsLayout ws_layout(int32_t R) {
+    WsLayout L;
+    size_t o = 0;
+    L.hmax = o;
+    o = align256(o + 256);
+    const size_t n = (size_t)R * R;
+    L.grad_n = o;
+    o = align256(o + sizeof(float) * 3 * n);
+    L.grad_z = o;
+    o = align256(o + sizeof(float) * n);
+    L.grad_g = o;
+    o = align256(o + sizeof(float) * 2 * n);
+    L.end = o;
+    return L;
+}
+
+// ---------------------------------------------------------------------------
+// Sobel normals (model/caustics.py:215-221): replicate padding, kernels
+// [[-1,0,1],[-2,0,2],[-1,0,1]] (x) and its transpose (y), scaled by
+// -1/(8 delta), z = 1, normalised.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void sobel_raw(const float* __restrict__ h, int R, int r, int c, float& gx, float& gy) {
+    const int r0 = max(r - 1, 0), r2 = min(r + 1, R - 1), c0 = max(c - 1, 0), c2 = min(c + 1, R - 1);
+    const float* a = h + r0 * R;
+    const float* b = h + r * R;
+    const float* d = h + r2 * R;
+    gx = (a[c2] - a[c0]) + 2.0f * (b[c2] - b[c0]) + (d[c2] - d[c0]);
+    gy = (d[c0] - a[c0]) + 2.0f * (d[c] - a[c]) + (d[c2] - a[c2]);
+}
+
+__global__ __launch_bounds__(256) void normals_kernel(const float* __restrict__ h, int R, float s,
+                                                      float* __restrict__ n) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= R * R) return;
+    const int r = k / R, c = k - r * R;
+    float gx, gy;
+    sobel_raw(h, R, r, c, gx, gy);
+    const V3 v = normalize(v3(gx * s, gy * s, 1.0f));
+    n[3 * k] = v.x;
+    n[3 * k + 1] = v.y;
+    n[3 * k + 2] = v.z;
+}
+
+// max(h) into ws[0], one workgroup
+__global__ __launch_bounds__(1024) void hmax_kernel(const float* __restrict__ h, int64_t n, float* __restrict__ out) {
+    __shared__ float part[16];
+    float m = -INFINITY;
+    for (int64_t k = threadIdx.x; k < n; k += 1024) m = fmaxf(m, h[k]);
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float r = -INFINITY;
+        for (int w = 0; w < 16; ++w) r = fmaxf(r, part[w]);
+        out[0] = r;
+    }
+}
+
+SceneArgs scene_args(const float* h, const float* normals, const hf_scene* sc, const float* hmax_dev) {
+    SceneArgs S;
+    S.h = h;
+    S.normals = normals;
+    S.R = sc->R;
+    S.delta = (float)(2.0 / (double)(sc->R - 1));
+    S.inv_delta = (float)((double)(sc->R - 1) / 2.0);
+    S.sensor_z = sc->sensor_z;
+    S.hmax = 0.0f;  // read from hmax_dev on the device
+    (void)hmax_dev;
+    return S;
+}
+
+// ---------------------------------------------------------------------------
+// trace contract
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void trace_kernel(SceneArgs S, const float* __restrict__ hmax,
+                                                    const float* __restrict__ org, const float* __restrict__ dir,
+                                                    int64_t N, float* __restrict__ t, float* __restrict__ uv,
+                                                    int64_t* __restrict__ obj, int64_t* __restrict__ prim) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= N) return;
+    S.hmax = hmax[0];
+    const V3 o = v3(org[3 * i], org[3 * i + 1], org[3 * i + 2]);
+    const V3 d = v3(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]);
+    const Hit hit = trace(S, o, d);
+    t[i] = hit.obj >= 0 ? hit.t : kMissT;
+    uv[2 * i] = hit.obj >= 0 ? hit.b1 : -1.0f;
+    uv[2 * i + 1] = hit.obj >= 0 ? hit.b2 : -1.0f;
+    obj[i] = hit.obj;
+    prim[i] = hit.obj >= 0 ? hit.prim : -1;
+}
+
+// ---------------------------------------------------------------------------
+// the photon path
+// ---------------------------------------------------------------------------
+struct EmitArgs {
+    int32_t n_side, channels;
+    float z0, pitch, flux;
+    float ior[HF_MAX_CHANNELS];
+};
+
+struct PathState {
+    V3 o, d, Du, Dv, Dt, Dp;
+};
+
+__device__ __forceinline__ PathState emit(const EmitArgs& E, int64_t i, int& ch) {
+    ch = (int)(i % E.channels);
+    const int64_t cell = i / E.channels;
+    const int row = (int)(cell / E.n_side), col = (int)(cell - (int64_t)row * E.n_side);
+    PathState s;
+    s.o = v3(-1.0f + (float)(2 * col + 1) / (float)E.n_side, -1.0f + (float)(2 * row + 1) / (float)E.n_side, E.z0);
+    s.d = v3(0.f, 0.f, -1.f);
+    s.Du = v3(2.0f * E.pitch, 0.f, 0.f);
+    s.Dv = v3(0.f, 2.0f * E.pitch, 0.f);
+    s.Dt = v3(0.f, 0.f, 0.f);
+    s.Dp = v3(0.f, 0.f, 0.f);
+    return s;
+}
+
+// model/photon_differential.py:70-78: Du' = Du + t Dth - beta d,
+// beta = ng . (Du + t Dth) / (ng . d)
+__device__ __forceinline__ V3 transfer1(V3 ng, V3 d, float t, V3 Du, V3 Dt) {
+    const V3 X = Du + t * Dt;
+    const float beta = dot(ng, X) / dot(ng, d);
+    return X - beta * d;
+}
+
+// refract_reflect_differentials (model/caustics.py:152-212) with dN = 0:
+// cosi = clamp(d . ns) unflipped, N = +-ns, eta = etai / etat.
+struct Refr {
+    V3 N;
+    float cosi, eta, sq, coso, a;
+    bool flip, tir, clamped;
+};
+
+__device__ __forceinline__ Refr refr_setup(V3 d, V3 ns, float eo, float ei) {
+    Refr R;
+    const float raw = dot(d, ns);
+    R.clamped = !(raw >= -1.0f && raw <= 1.0f);
+    R.cosi = fminf(fmaxf(raw, -1.0f), 1.0f);
+    R.flip = R.cosi > 0.0f;
+    R.N = R.flip ? -ns : ns;
+    R.eta = R.flip ? ei / eo : eo / ei;
+    const float k = 1.0f - R.eta * R.eta * (1.0f - R.cosi * R.cosi);
+    R.tir = !(k > 0.0f);
+    R.sq = R.tir ? 0.0f : sqrtf(k);
+    R.coso = R.eta * R.cosi + R.sq;
+    R.a = R.tir ? 0.0f : R.eta + R.eta * R.eta * R.cosi / R.sq;
+    return R;
+}
+
+__device__ __forceinline__ void refract_apply(const Refr& R, V3& d, V3& Dt, V3& Dp) {
+    if (!R.tir) {
+        const V3 dn = R.eta * d - R.coso * R.N;
+        Dt = R.eta * Dt - (R.a * dot(Dt, R.N)) * R.N;
+        Dp = R.eta * Dp - (R.a * dot(Dp, R.N)) * R.N;
+        d = dn;
+    } else {
+        const V3 dn = d - (2.0f * R.cosi) * R.N;
+        Dt = Dt - (2.0f * dot(Dt, R.N)) * R.N;
+        Dp = Dp - (2.0f * dot(Dp, R.N)) * R.N;
+        d = dn;
+    }
+}
+
+struct Splat {
+    float Ep, x, y, radius;
+    V3 M0, M1;
+    bool good;
+};
+
+// model/caustics.py:224-239
+__device__ __forceinline__ Splat splat_inputs(V3 p, V3 Du, V3 Dv, V3 ns, float flux) {
+    Splat s;
+    const V3 tmp = cross(Dv, ns);
+    const float den = dot(Du, tmp);
+    s.good = den != 0.0f;
+    const float f = 2.0f / den;
+    s.M0 = f * tmp;
+    s.M1 = f * cross(ns, Du);
+    s.radius = 0.5f * fmaxf(norm(Du), norm(Dv));
+    s.Ep = flux / norm(cross(Du, Dv)) * 4.0f;
+    s.x = p.x;
+    s.y = p.y;
+    return s;
+}
+
+struct OutArgs {
+    float *Ep, *xp, *Mp, *radius;
+    int64_t* cp;
+    int64_t N;
+};
+
+__device__ __forceinline__ void write_dropped(const OutArgs& O, int64_t i) {
+    const int64_t N = O.N;
+    O.Ep[i] = 0.0f;
+    O.xp[i] = 0.0f;
+    O.xp[N + i] = 0.0f;
+    for (int k = 0; k < 6; ++k) O.Mp[k * N + i] = 0.0f;
+    O.radius[i] = 0.0f;
+    O.cp[i] = -1;
+}
+
+__global__ __launch_bounds__(256) void simulate_kernel(SceneArgs S, const float* __restrict__ hmax, EmitArgs E,
+                                                       OutArgs O) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= O.N) return;
+    S.hmax = hmax[0];
+    int ch;
+    PathState s = emit(E, i, ch);
+    const float ei = E.ior[ch], eo = 1.0f;
+    for (int bounce = 0; bounce < kMaxBounces; ++bounce) {
+        const Hit hit = trace(S, s.o, s.d);
+        if (hit.obj < 0) break;  // miss: advance_path drops it
+        const float t = hit.t;
+        const V3 p = s.o + t * s.d;
+        const TriVerts T = tri_verts(S, hit.obj, hit.prim);
+        const V3 ng = geometric_normal(T);
+        const V3 ns = normalize(shading_raw(T, hit.b1, hit.b2));
+        const V3 Du = transfer1(ng, s.d, t, s.Du, s.Dt);
+        const V3 Dv = transfer1(ng, s.d, t, s.Dv, s.Dp);
+        if (hit.obj == 1) {
+            const Splat sp = splat_inputs(p, Du, Dv, ns, E.flux);
+            if (!sp.good) break;
+            const int64_t N = O.N;
+            O.Ep[i] = sp.Ep;
+            O.xp[i] = sp.x;
+            O.xp[N + i] = sp.y;
+            O.Mp[i] = sp.M0.x;
+            O.Mp[N + i] = sp.M0.y;
+            O.Mp[2 * N + i] = sp.M0.z;
+            O.Mp[3 * N + i] = sp.M1.x;
+            O.Mp[4 * N + i] = sp.M1.y;
+            O.Mp[5 * N + i] = sp.M1.z;
+            O.radius[i] = sp.radius;
+            O.cp[i] = ch;
+            return;
+        }
+        const Refr R = refr_setup(s.d, ns, eo, ei);
+        refract_apply(R, s.d, s.Dt, s.Dp);
+        s.Du = Du;
+        s.Dv = Dv;
+        s.o = p;
+    }
+    write_dropped(O, i);
+}
+
+// ---------------------------------------------------------------------------
+// adjoint
+// ---------------------------------------------------------------------------
+struct Tape {
+    V3 d, Du, Dv, Dt, Dp;
+    float t, b1, b2;
+    int obj, prim;
+};
+
+__device__ __forceinline__ void atomic_add3(float* p, V3 v) {
+    unsafeAtomicAdd(p, v.x);
+    unsafeAtomicAdd(p + 1, v.y);
+    unsafeAtomicAdd(p + 2, v.z);
+}
+
+// adjoint of Du' = X - beta d, X = Du + t Dt, beta = ng . X / (ng . d):
+// adds to gDu, gDt, gd and (optionally) gng
+__device__ __forceinline__ void transfer1_adj(V3 ng, V3 d, float t, V3 Du, V3 Dt, V3 g, V3& gDu, V3& gDt, V3& gd,
+                                              V3& gng) {
+    const V3 X = Du + t * Dt;
+    const float den = dot(ng, d);
+    const float beta = dot(ng, X) / den;
+    const float gd_ = dot(g, d);
+    // d(Du') = dX - dbeta d - beta dd;  dbeta = (ng.dX + X.dng - beta (ng.dd + d.dng)) / den
+    const V3 gX = g - (gd_ / den) * ng;
+    gDu = gDu + gX;
+    gDt = gDt + t * gX;
+    gd = gd + (-beta) * g + (gd_ * beta / den) * ng;
+    gng = gng + (-gd_ / den) * (X - beta * d);
+}
+
+__global__ __launch_bounds__(256) void adjoint_kernel(SceneArgs S, const float* __restrict__ hmax, EmitArgs E,
+                                                      int64_t N, const float* __restrict__ gEp,
+                                                      const float* __restrict__ gxp, const float* __restrict__ gMp,
+                                                      float* __restrict__ grad_n, float* __restrict__ grad_z) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= N) return;
+    S.hmax = hmax[0];
+    int ch;
+    PathState s = emit(E, i, ch);
+    const float ei = E.ior[ch], eo = 1.0f;
+    Tape tape[kMaxBounces];
+    int nb = 0;
+    bool reached = false;
+    // forward, recording
+    for (int bounce = 0; bounce < kMaxBounces; ++bounce) {
+        const Hit hit = trace(S, s.o, s.d);
+        if (hit.obj < 0) break;
+        Tape& tp = tape[nb++];
+        tp.d = s.d;
+        tp.Du = s.Du;
+        tp.Dv = s.Dv;
+        tp.Dt = s.Dt;
+        tp.Dp = s.Dp;
+        tp.t = hit.t;
+        tp.b1 = hit.b1;
+        tp.b2 = hit.b2;
+        tp.obj = hit.obj;
+        tp.prim = hit.prim;
+        const float t = hit.t;
+        const V3 p = s.o + t * s.d;
+        const TriVerts T = tri_verts(S, hit.obj, hit.prim);
+        const V3 ng = geometric_normal(T);
+        const V3 ns = normalize(shading_raw(T, hit.b1, hit.b2));
+        const V3 Du = transfer1(ng, s.d, t, s.Du, s.Dt);
+        const V3 Dv = transfer1(ng, s.d, t, s.Dv, s.Dp);
+        if (hit.obj == 1) {
+            const Splat sp = splat_inputs(p, Du, Dv, ns, E.flux);
+            reached = sp.good;
+            break;
+        }
+        const Refr R = refr_setup(s.d, ns, eo, ei);
+        refract_apply(R, s.d, s.Dt, s.Dp);
+        s.Du = Du;
+        s.Dv = Dv;
+        s.o = p;
+    }
+    if (!reached) return;
+    // upstream gradients of the splat inputs
+    const float gE = gEp[i];
+    const float gx0 = gxp[i], gx1 = gxp[N + i];
+    const V3 gM0 = v3(gMp[i], gMp[N + i], gMp[2 * N + i]);
+    const V3 gM1 = v3(gMp[3 * N + i], gMp[4 * N + i], gMp[5 * N + i]);
+    // ---- last bounce: ground plane (constant geometry)
+    const Tape& L = tape[nb - 1];
+    V3 gp = v3(gx0, gx1, 0.0f), gd, gDu, gDv, gDt, gDp;
+    {
+        const TriVerts T = tri_verts(S, L.obj, L.prim);
+        const V3 ng = geometric_normal(T);
+        const V3 ns = normalize(shading_raw(T, L.b1, L.b2));
+        const V3 Du = transfer1(ng, L.d, L.t, L.Du, L.Dt);
+        const V3 Dv = transfer1(ng, L.d, L.t, L.Dv, L.Dp);
+        // splat adjoint (ns constant)
+        const V3 tmp = cross(Dv, ns), sv = cross(ns, Du), c = cross(Du, Dv);
+        const float den = dot(Du, tmp), f = 2.0f / den;
+        const float Lc = norm(c), Ep = E.flux / Lc * 4.0f;
+        const V3 gc = (-Ep / (Lc * Lc) * gE) * c;
+        V3 gDu2 = cross(Dv, gc), gDv2 = cross(gc, Du);
+        V3 gtmp = f * gM0;
+        const V3 gs = f * gM1;
+        const float gf = dot(gM0, tmp) + dot(gM1, sv);
+        const float gden = -(2.0f / (den * den)) * gf;
+        gDu2 = gDu2 + gden * tmp;
+        gtmp = gtmp + gden * Du;
+        gDv2 = gDv2 + cross(ns, gtmp);
+        gDu2 = gDu2 + cross(gs, ns);
+        // transfer adjoint at the ground (ng constant)
+        gDu = v3(0.f, 0.f, 0.f), gDv = gDu, gDt = gDu, gDp = gDu, gd = gDu;
+        V3 gng = gDu;
+        transfer1_adj(ng, L.d, L.t, L.Du, L.Dt, gDu2, gDu, gDt, gd, gng);
+        transfer1_adj(ng, L.d, L.t, L.Dv, L.Dp, gDv2, gDv, gDp, gd, gng);
+        // p = o + t d
+        gd = gd + L.t * gp;
+    }
+    // ---- earlier bounces, newest first: gradients arrive for (o' = p, d', Du', Dv', Dt', Dp')
+    for (int k = nb - 2; k >= 0; --k) {
+        const Tape& B = tape[k];
+        const TriVerts T = tri_verts(S, B.obj, B.prim);
+        const V3 graw = cross(T.v[1] - T.v[0], T.v[2] - T.v[0]);
+        const float glen = norm(graw);
+        const V3 ng = glen > 0.0f ? (1.0f / glen) * graw : v3(0.f, 0.f, 0.f);
+        const float w0 = 1.0f - B.b1 - B.b2;
+        const V3 m = shading_raw(T, B.b1, B.b2);
+        const float mlen = norm(m);
+        const V3 ns = mlen > 0.0f ? (1.0f / mlen) * m : v3(0.f, 0.f, 0.f);
+        const Refr R = refr_setup(B.d, ns, eo, ei);
+        // refraction adjoint: outputs d', Dt', Dp' (gradients gd, gDt, gDp)
+        V3 gd_in = v3(0.f, 0.f, 0.f), gDt_in = gd_in, gDp_in = gd_in, gN = gd_in;
+        float gcosi = 0.0f;
+        if (!R.tir) {
+            gd_in = R.eta * gd;
+            const float gcoso = -dot(gd, R.N);
+            gN = gN + (-R.coso) * gd;
+            const float tN = dot(B.Dt, R.N), gtN = dot(gDt, R.N);
+            const float pN = dot(B.Dp, R.N), gpN = dot(gDp, R.N);
+            gDt_in = R.eta * gDt - (R.a * gtN) * R.N;
+            gDp_in = R.eta * gDp - (R.a * gpN) * R.N;
+            const float ga = -(tN * gtN) - (pN * gpN);
+            gN = gN + (-R.a) * (gtN * B.Dt + tN * gDt) + (-R.a) * (gpN * B.Dp + pN * gDp);
+            const float e2 = R.eta * R.eta;
+            const float dadc = e2 * (1.0f - e2) / (R.sq * R.sq * R.sq);
+            gcosi = R.a * gcoso + dadc * ga;
+        } else {
+            gd_in = gd;
+            gcosi = -2.0f * dot(gd, R.N);
+            gN = gN + (-2.0f * R.cosi) * gd;
+            const float tN = dot(B.Dt, R.N), gtN = dot(gDt, R.N);
+            const float pN = dot(B.Dp, R.N), gpN = dot(gDp, R.N);
+            gDt_in = gDt - (2.0f * gtN) * R.N;
+            gDp_in = gDp - (2.0f * gpN) * R.N;
+            gN = gN + (-2.0f) * (gtN * B.Dt + tN * gDt) + (-2.0f) * (gpN * B.Dp + pN * gDp);
+        }
+        V3 gns = R.flip ? -gN : gN;
+        if (!R.clamped) {
+            gd_in = gd_in + gcosi * ns;
+            gns = gns + gcosi * B.d;
+        }
+        // transfer adjoint (Du', Dv' <- Du, Dv, Dt, Dp, d, ng)
+        V3 gDu_in = v3(0.f, 0.f, 0.f), gDv_in = gDu_in, gng = gDu_in;
+        transfer1_adj(ng, B.d, B.t, B.Du, B.Dt, gDu, gDu_in, gDt_in, gd_in, gng);
+        transfer1_adj(ng, B.d, B.t, B.Dv, B.Dp, gDv, gDv_in, gDp_in, gd_in, gng);
+        // p = o + t d: gradient of o is gp (passed on), of d is t gp
+        gd_in = gd_in + B.t * gp;
+        // shading normal -> vertex normals (top-surface vertices only)
+        if (mlen > 0.0f) {
+            const V3 gm = (1.0f / mlen) * (gns - dot(gns, ns) * ns);
+            const float w[3] = {w0, B.b1, B.b2};
+            for (int v = 0; v < 3; ++v)
+                if (T.vr[v] >= 0 && B.prim < prim_base(S.R).bottom)
+                    atomic_add3(grad_n + 3 * (T.vr[v] * S.R + T.vc[v]), w[v] * gm);
+        }
+        // geometric normal -> vertex heights
+        if (glen > 0.0f) {
+            const V3 gg = (1.0f / glen) * (gng - dot(gng, ng) * ng);
+            const V3 e1 = T.v[1] - T.v[0], e2 = T.v[2] - T.v[0];
+            const V3 ge1 = cross(e2, gg), ge2 = cross(gg, e1);
+            const float gz[3] = {-ge1.z - ge2.z, ge1.z, ge2.z};
+            for (int v = 0; v < 3; ++v)
+                if (T.vr[v] >= 0) unsafeAtomicAdd(grad_z + T.vr[v] * S.R + T.vc[v], gz[v]);
+        }
+        gd = gd_in;
+        gDu = gDu_in;
+        gDv = gDv_in;
+        gDt = gDt_in;
+        gDp = gDp_in;
+    }
+}
+
+__global__ __launch_bounds__(256) void zero_kernel(float* __restrict__ p, int64_t n) {
+    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (k < n) p[k] = 0.0f;
+}
+
+// per vertex: gradient of the raw Sobel sums (gx, gy) from the gradient of
+// the normalised normal: n = v / |v|, v = (s gx, s gy, 1)
+__global__ __launch_bounds__(256) void normals_adj_kernel(const float* __restrict__ h, int R, float s,
+                                                          const float* __restrict__ grad_n,
+                                                          float* __restrict__ grad_g) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= R * R) return;
+    const int r = k / R, c = k - r * R;
+    float gx, gy;
+    sobel_raw(h, R, r, c, gx, gy);
+    const V3 v = v3(gx * s, gy * s, 1.0f);
+    const float len = norm(v);
+    const V3 n = (1.0f / len) * v;
+    const V3 g = v3(grad_n[3 * k], grad_n[3 * k + 1], grad_n[3 * k + 2]);
+    const V3 gv = (1.0f / len) * (g - dot(g, n) * n);
+    grad_g[2 * k] = s * gv.x;
+    grad_g[2 * k + 1] = s * gv.y;
+}
+
+// dL/dh[r, c] = grad_z[r, c] + sum over the vertices whose (clamped) Sobel
+// stencil reads h[r, c] of the stencil weight times their (ggx, ggy)
+__global__ __launch_bounds__(256) void grad_h_kernel(int R, const float* __restrict__ grad_g,
+                                                     const float* __restrict__ grad_z, float* __restrict__ grad_h) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= R * R) return;
+    const int r = k / R, c = k - r * R;
+    float acc = grad_z[k];
+    const float w[3] = {1.0f, 2.0f, 1.0f};
+    for (int vr = max(r - 1, 0); vr <= min(r + 1, R - 1); ++vr)
+        for (int vc = max(c - 1, 0); vc <= min(c + 1, R - 1); ++vc) {
+            const float ggx = grad_g[2 * (vr * R + vc)], ggy = grad_g[2 * (vr * R + vc) + 1];
+            for (int a = 0; a < 3; ++a) {
+                // x stencil: rows vr - 1 + a (clamped), columns vc +- 1 (clamped)
+                const int rr = min(max(vr - 1 + a, 0), R - 1);
+                if (rr == r) {
+                    if (min(vc + 1, R - 1) == c) acc += w[a] * ggx;
+                    if (max(vc - 1, 0) == c) acc -= w[a] * ggx;
+                }
+                // y stencil: columns vc - 1 + a (clamped), rows vr +- 1 (clamped)
+                const int cc = min(max(vc - 1 + a, 0), R - 1);
+                if (cc == c) {
+                    if (min(vr + 1, R - 1) == r) acc += w[a] * ggy;
+                    if (max(vr - 1, 0) == r) acc -= w[a] * ggy;
+                }
+            }
+        }
+    grad_h[k] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// Landweber data term and update
+// ---------------------------------------------------------------------------
+constexpr int kRedBlocks = 256;
+
+__global__ __launch_bounds__(256) void residual_kernel(const float* __restrict__ img, const float* __restrict__ tgt,
+                                                       int64_t n, float* __restrict__ resid,
+                                                       float* __restrict__ partial) {
+    __shared__ float part[4];
+    float acc = 0.0f;
+    for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (int64_t)gridDim.x * 256) {
+        const float r = img[k] - tgt[k];
+        resid[k] = r;
+        acc = fmaf(r, r, acc);
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = (part[0] + part[1]) + (part[2] + part[3]);
+}
+
+__global__ __launch_bounds__(256) void sum_partials_kernel(const float* __restrict__ partial, int n, float scale,
+                                                           float* __restrict__ out) {
+    __shared__ float part[4];
+    float acc = 0.0f;
+    for (int k = threadIdx.x; k < n; k += 256) acc += partial[k];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) out[0] = scale * ((part[0] + part[1]) + (part[2] + part[3]));
+}
+
+struct LwArgs {
+    float tau, alpha, lower, upper, d1, d2, gamma;
+    int32_t radius;
+};
+
+// x = mask * extShrink(d - tau g, tau alpha, lower, upper), written to x;
+// per-block partial sums of x
+__global__ __launch_bounds__(256) void landweber_step_kernel(const float* __restrict__ d,
+                                                             const float* __restrict__ g,
+                                                             const float* __restrict__ mask, int64_t n, LwArgs A,
+                                                             float* __restrict__ x, float* __restrict__ partial) {
+    __shared__ float part[4];
+    float acc = 0.0f;
+    for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (int64_t)gridDim.x * 256) {
+        const float y = d[k] - A.tau * g[k];
+        const float sgn = y > 0.0f ? 1.0f : (y < 0.0f ? -1.0f : 0.0f);
+        float v = fmaxf(fabsf(y) - A.tau * A.alpha, 0.0f) * sgn;
+        v = fmaxf(fminf(v, A.upper), A.lower);
+        v *= mask[k];
+        x[k] = v;
+        acc += v;
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = (part[0] + part[1]) + (part[2] + part[3]);
+}
+
+// VconsHeur: box mean of x over (2 radius + 1)^2 (zero padding, divisor the
+// full window, as avg_pool2d counts padding), then x +- mask mean sin(..) gamma
+__global__ __launch_bounds__(256) void vcons_kernel(const float* __restrict__ x, const float* __restrict__ mask,
+                                                    int R, LwArgs A, const float* __restrict__ partial, int nparts,
+                                                    float* __restrict__ d) {
+    __shared__ float vx_s;
+    if (threadIdx.x == 0) {
+        float acc = 0.0f;
+        for (int k = 0; k < nparts; ++k) acc += partial[k];
+        vx_s = acc;
+    }
+    __syncthreads();
+    const float vx = vx_s;
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= R * R) return;
+    const int r = k / R, c = k - r * R;
+    float f = 0.0f;
+    if (vx <= A.d1)
+        f = sinf(vx / A.d1 * 3.14159265358979f) * A.gamma;
+    else if (vx >= A.d2)
+        f = -sinf((vx / A.d2 - 1.0f) * 3.14159265358979f) * A.gamma;
+    float out = x[k];
+    if (f != 0.0f) {
+        float s = 0.0f;
+        for (int rr = r - A.radius; rr <= r + A.radius; ++rr)
+            for (int cc = c - A.radius; cc <= c + A.radius; ++cc)
+                if (rr >= 0 && rr < R && cc >= 0 && cc < R) s += x[rr * R + cc];
+        const int w = 2 * A.radius + 1;
+        out += mask[k] * (s / (float)(w * w)) * f;
+    }
+    d[k] = out;
+}
+
+unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+EmitArgs emit_args(const hf_scene* sc, const hf_emitter* em) {
+    EmitArgs E;
+    E.n_side = em->n_side;
+    E.channels = sc->channels;
+    E.z0 = em->z0;
+    E.pitch = (float)(2.0 / (double)em->n_side);
+    E.flux = (float)((double)em->flux_total / ((double)em->n_side * (double)em->n_side));
+    for (int k = 0; k < HF_MAX_CHANNELS; ++k) E.ior[k] = sc->ior[k];
+    return E;
+}
+
+int check_scene(const hf_scene* sc, const void* h, const void* normals) {
+    if (!sc) return fail(PDS_ERR_ARGUMENT, "null scene");
+    if (sc->R < 2 || sc->R > 32768) return fail(PDS_ERR_ARGUMENT, "scene R must be in [2, 32768]");
+    if (sc->channels < 1 || sc->channels > HF_MAX_CHANNELS) return fail(PDS_ERR_ARGUMENT, "channels must be in [1, 8]");
+    if (!h || !normals) return fail(PDS_ERR_ARGUMENT, "null height field or normals");
+    return PDS_OK;
+}
+
+int check_emitter(const hf_emitter* em) {
+    if (!em) return fail(PDS_ERR_ARGUMENT, "null emitter");
+    if (em->n_side < 1 || em->n_side > 65536) return fail(PDS_ERR_ARGUMENT, "emitter n_side must be in [1, 65536]");
+    return PDS_OK;
+}
+
+int hip_status(const char* what) {
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        snprintf(g_err, sizeof(g_err), "%s: %s", what, hipGetErrorString(e));
+        return PDS_ERR_HIP;
+    }
+    return PDS_OK;
+}
+
+}  // namespace
+
+}  // namespace hf
+
+using namespace hf;
+
+extern "C" {
+
+const char* hf_last_error(void) { return g_err; }
+
+size_t hf_workspace_bytes(int32_t R) { return R < 2 ? 0 : ws_layout(R).end; }
+
+int hf_normals_f32(const float* h, int32_t R, float* normals, void* stream) {
+    g_err[0] = '\0';
+    if (R < 2 || !h || !normals) return fail(PDS_ERR_ARGUMENT, "invalid height field");
+    const float s = (float)(-1.0 / (8.0 * (2.0 / (double)(R - 1))));
+    hipLaunchKernelGGL(normals_kernel, dim3(blocks_for((int64_t)R * R)), dim3(256), 0, (hipStream_t)stream, h, R, s,
+                       normals);
+    return hip_status("hf_normals");
+}
+
+int hf_trace_f32(const float* h, const float* normals, const hf_scene* sc, const float* origins, const float* dirs,
+                 int64_t N, float* t, float* uv, int64_t* object, int64_t* primitive, void* ws, size_t ws_bytes,
+                 void* stream) {
+    g_err[0] = '\0';
+    int rc = check_scene(sc, h, normals);
+    if (rc) return rc;
+    if (N < 0) return fail(PDS_ERR_ARGUMENT, "negative ray count");
+    if (N == 0) return PDS_OK;
+    if (!origins || !dirs || !t || !uv || !object || !primitive) return fail(PDS_ERR_ARGUMENT, "null ray buffer");
+    const WsLayout L = ws_layout(sc->R);
+    if (!ws || ws_bytes < L.end) return fail(PDS_ERR_WORKSPACE, "workspace smaller than hf_workspace_bytes(R)");
+    float* hmax = (float*)((char*)ws + L.hmax);
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(hmax_kernel, dim3(1), dim3(1024), 0, s, h, (int64_t)sc->R * sc->R, hmax);
+    hipLaunchKernelGGL(trace_kernel, dim3(blocks_for(N)), dim3(256), 0, s, scene_args(h, normals, sc, hmax),
+                       (const float*)hmax, origins, dirs, N, t, uv, object, primitive);
+    return hip_status("hf_trace");
+}
+
+int hf_simulate_f32(const float* h, const float* normals, const hf_scene* sc, const hf_emitter* em, float* Ep,
+                    float* xp, float* Mp, int64_t* cp, float* radius, void* ws, size_t ws_bytes, void* stream) {
+    g_err[0] = '\0';
+    int rc = check_scene(sc, h, normals);
+    if (rc) return rc;
+    if ((rc = check_emitter(em))) return rc;
+    if (!Ep || !xp || !Mp || !cp || !radius) return fail(PDS_ERR_ARGUMENT, "null output buffer");
+    const WsLayout L = ws_layout(sc->R);
+    if (!ws || ws_bytes < L.end) return fail(PDS_ERR_WORKSPACE, "workspace smaller than hf_workspace_bytes(R)");
+    float* hmax = (float*)((char*)ws + L.hmax);
+    const int64_t N = (int64_t)em->n_side * em->n_side * sc->channels;
+    OutArgs O{Ep, xp, Mp, radius, cp, N};
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(hmax_kernel, dim3(1), dim3(1024), 0, s, h, (int64_t)sc->R * sc->R, hmax);
+    hipLaunchKernelGGL(simulate_kernel, dim3(blocks_for(N)), dim3(256), 0, s, scene_args(h, normals, sc, hmax),
+                       (const float*)hmax, emit_args(sc, em), O);
+    return hip_status("hf_simulate");
+}
+
+int hf_adjoint_f32(const float* h, const float* normals, const hf_scene* sc, const hf_emitter* em,
+                   const float* grad_Ep, const float* grad_xp, const float* grad_Mp, float* grad_h, void* ws,
+                   size_t ws_bytes, void* stream) {
+    g_err[0] = '\0';
+    int rc = check_scene(sc, h, normals);
+    if (rc) return rc;
+    if ((rc = check_emitter(em))) return rc;
+    if (!grad_Ep || !grad_xp || !grad_Mp || !grad_h) return fail(PDS_ERR_ARGUMENT, "null gradient buffer");
+    const WsLayout L = ws_layout(sc->R);
+    if (!ws || ws_bytes < L.end) return fail(PDS_ERR_WORKSPACE, "workspace smaller than hf_workspace_bytes(R)");
+    char* b = (char*)ws;
+    float* hmax = (float*)(b + L.hmax);
+    float* gn = (float*)(b + L.grad_n);
+    float* gz = (float*)(b + L.grad_z);
+    float* gg = (float*)(b + L.grad_g);
+    const int R = sc->R;
+    const int64_t RR = (int64_t)R * R;
+    const int64_t N = (int64_t)em->n_side * em->n_side * sc->channels;
+    const float s_sobel = (float)(-1.0 / (8.0 * (2.0 / (double)(R - 1))));
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(zero_kernel, dim3(blocks_for(3 * RR)), dim3(256), 0, s, gn, 3 * RR);
+    hipLaunchKernelGGL(zero_kernel, dim3(blocks_for(RR)), dim3(256), 0, s, gz, RR);
+    hipLaunchKernelGGL(hmax_kernel, dim3(1), dim3(1024), 0, s, h, RR, hmax);
+    hipLaunchKernelGGL(adjoint_kernel, dim3(blocks_for(N)), dim3(256), 0, s, scene_args(h, normals, sc, hmax),
+                       (const float*)hmax, emit_args(sc, em), N, grad_Ep, grad_xp, grad_Mp, gn, gz);
+    hipLaunchKernelGGL(normals_adj_kernel, dim3(blocks_for(RR)), dim3(256), 0, s, h, R, s_sobel, (const float*)gn, gg);
+    hipLaunchKernelGGL(grad_h_kernel, dim3(blocks_for(RR)), dim3(256), 0, s, R, (const float*)gg, (const float*)gz,
+                       grad_h);
+    return hip_status("hf_adjoint");
+}
+
+size_t hf_residual_workspace_bytes(int64_t n) { return n < 0 ? 0 : sizeof(float) * kRedBlocks; }
+
+int hf_residual_f32(const float* img, const float* target, int64_t n, float* resid, float* loss, void* ws,
+                    size_t ws_bytes, void* stream) {
+    g_err[0] = '\0';
+    if (n < 0 || (n > 0 && (!img || !target || !resid)) || !loss) return fail(PDS_ERR_ARGUMENT, "invalid residual args");
+    if (!ws || ws_bytes < sizeof(float) * kRedBlocks) return fail(PDS_ERR_WORKSPACE, "residual workspace too small");
+    hipStream_t s = (hipStream_t)stream;
+    float* partial = (float*)ws;
+    hipLaunchKernelGGL(residual_kernel, dim3(kRedBlocks), dim3(256), 0, s, img, target, n, resid, partial);
+    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, s, (const float*)partial, kRedBlocks, 0.5f, loss);
+    return hip_status("hf_residual");
+}
+
+size_t hf_landweber_workspace_bytes(int32_t R) {
+    return R < 1 ? 0 : align256(sizeof(float) * kRedBlocks) + sizeof(float) * (size_t)R * R;
+}
+
+int hf_landweber_f32(float* d, const float* grad, const float* mask, int32_t R, const hf_landweber* p, void* ws,
+                     size_t ws_bytes, void* stream) {
+    g_err[0] = '\0';
+    if (R < 1 || !d || !grad || !mask || !p) return fail(PDS_ERR_ARGUMENT, "invalid Landweber args");
+    if (p->volume_radius < 0 || p->volume_radius > 64) return fail(PDS_ERR_ARGUMENT, "volume_radius must be in [0, 64]");
+    if (!ws || ws_bytes < hf_landweber_workspace_bytes(R)) return fail(PDS_ERR_WORKSPACE, "Landweber workspace too small");
+    hipStream_t s = (hipStream_t)stream;
+    float* partial = (float*)ws;
+    float* x = (float*)((char*)ws + align256(sizeof(float) * kRedBlocks));
+    LwArgs A{p->tau, p->alpha, p->lower, p->upper, p->d1, p->d2, p->gamma, p->volume_radius};
+    const int64_t n = (int64_t)R * R;
+    hipLaunchKernelGGL(landweber_step_kernel, dim3(kRedBlocks), dim3(256), 0, s, (const float*)d, grad, mask, n, A, x,
+                       partial);
+    hipLaunchKernelGGL(vcons_kernel, dim3(blocks_for(n)), dim3(256), 0, s, (const float*)x, mask, R, A,
+                       (const float*)partial, kRedBlocks, d);
+    return hip_status("hf_landweber");
+}
+
+}  // extern "C"
