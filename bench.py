@@ -79,6 +79,95 @@ def load_traffic(kernel, workload):
         return None
 
 
+def bench_landweber(args):
+    """One pixel-Landweber iteration of height-field reconstruction, end to end
+    on the GPU: Sobel normals, 2048^2 parallel rays traced and refracted
+    through a 256^2 height field (32 Gaussian bumps, 1 mm base, n = 1.5) to a
+    sensor 50 mm below, splatted into 1024^2 (cap 12); loss 1/2 |I - T|^2
+    against the image of a differently seeded field; PDS adjoint, path
+    adjoint (recomputed) and the masked shrinkage + volume update."""
+    import numpy as np
+    import torch
+
+    from shapefromcaustics_amd import heightfield as HF
+
+    R, n_side, S, cap, sz, base = 256, 2048, 1024, 12, -10.0, 0.2
+
+    def field(seed):
+        rng = np.random.Generator(np.random.PCG64(seed))
+        a, sg = rng.uniform(0.0, 0.1, 32), rng.uniform(0.06, 0.4, 32)
+        mu = rng.uniform(-0.6, 0.6, (32, 2))
+        lin = np.linspace(-1, 1, R)
+        X, Y = np.meshgrid(lin, lin)
+        h = np.full((R, R), base)
+        for k in range(32):
+            h += a[k] * np.exp(-((X - mu[k, 0]) ** 2 + (Y - mu[k, 1]) ** 2) / (2 * sg[k] ** 2))
+        return torch.tensor(h, dtype=torch.float32, device="cuda")
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    sc, em = HF.Scene(R, sz, (1.5,)), HF.ParallelEmitter(n_side, 2.0, 1.0)
+    size = (1, S, S)
+    target = HF.CausticImage.apply(field(5), sc, em, size, cap).detach()
+    d = (field(4) - base).contiguous()
+    mask = torch.ones_like(d)
+    params = HF.LandweberParams()
+    N = n_side * n_side
+
+    d0 = d.clone()
+
+    def step():
+        # every timed iteration starts from the same field (a 256^2 copy), so
+        # each measures the same physical problem
+        d.copy_(d0)
+        return HF.landweber_step(d, base, target, sc, em, size, cap, mask, params)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    t0 = time.perf_counter()
+    ev[0].record()
+    for _ in range(args.steps):
+        loss, img, gh = step()
+    ev[1].record()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / args.steps
+    gpu = ev[0].elapsed_time(ev[1]) * 1e-3 / args.steps
+    # stage split on one extra step
+    from shapefromcaustics_amd.PhotonDifferentialSplatting import _backward, _forward
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
+    d.copy_(d0)
+    h = (d + base).contiguous()
+    e[0].record()
+    n = HF.normals(h)
+    Ep, xp, Mp, cp, radius = HF.simulate(h, sc, em, n)
+    e[1].record()
+    img, bins = _forward(Ep, xp, Mp, cp, radius, size, cap, "auto", None, keep_bins=True)
+    resid, lossv = HF.residual(img, target)
+    e[2].record()
+    gE, gx, gM = _backward(resid, Ep, xp, Mp, cp, radius, cap, "auto", None, bins=bins)
+    e[3].record()
+    gh = HF.adjoint(h, sc, em, gE, gx, gM, n)
+    e[4].record()
+    HF.landweber_update(d, gh, mask, params)
+    e[5].record()
+    torch.cuda.synchronize()
+    names = ["normals+simulate", "splat+residual", "splat adjoint", "path adjoint", "update"]
+    stages = {k: round(e[i].elapsed_time(e[i + 1]) * 1e3, 1) for i, k in enumerate(names)}
+    log(f"[bench] landweber step {wall * 1e3:.3f} ms (GPU {gpu * 1e3:.3f} ms); stages (us) {stages}; "
+        f"loss {float(lossv):.4e}; photons reaching the sensor {int((cp >= 0).sum())} of {N}")
+    line = {
+        "metric": "Landweber steps/s (height-field reconstruction iteration, rays/s in config)",
+        "value": 1.0 / wall, "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded Gaussian-bump height fields)",
+        "config": {"workload": "landweber: 256^2 height field, 2048^2 parallel rays -> 1x1024x1024, cap 12, "
+                               "sensor 50 mm, n 1.5", "rays": N, "rays_per_s": N / wall, "stages_us": stages},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,6 +192,9 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (photon sharding plus one image "
               "all-reduce is the design, not built); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "landweber":
+        return bench_landweber(args)
 
     import numpy as np
     import torch

@@ -213,58 +213,60 @@ __device__ inline Hit trace(const SceneArgs& S, V3 o, V3 d) {
     mt_test(o, d, v3(-1.f, -1.f, S.sensor_z), v3(1.f, -1.f, S.sensor_z), v3(-1.f, 1.f, S.sensor_z), 1, 0, best);
     mt_test(o, d, v3(-1.f, 1.f, S.sensor_z), v3(1.f, -1.f, S.sensor_z), v3(1.f, 1.f, S.sensor_z), 1, 1, best);
     // bottom plane z = 0: the two triangles of the cell under the plane hit
+    // (its 3x3 neighbourhood only if rounding put the hit across a cell edge)
     if (d.z != 0.0f) {
         const float tp = -o.z / d.z;
         if (tp >= 0.0f) {
             const float x = fmaf(tp, d.x, o.x), y = fmaf(tp, d.y, o.y);
-            if (x >= -1.0f && x <= 1.0f && y >= -1.0f && y <= 1.0f) {
+            const float eps = 1e-5f;
+            if (x >= -1.0f - eps && x <= 1.0f + eps && y >= -1.0f - eps && y <= 1.0f + eps) {
                 const int c = clampi((int)floorf((x + 1.0f) * S.inv_delta), 0, e - 1);
                 const int r = clampi((int)floorf((y + 1.0f) * S.inv_delta), 0, e - 1);
-                // neighbours too: the plane hit may sit on a cell edge in float
-                for (int dr = -1; dr <= 1; ++dr)
-                    for (int dc = -1; dc <= 1; ++dc) {
-                        const int rr = r + dr, cc = c + dc;
-                        if (rr < 0 || rr >= e || cc < 0 || cc >= e) continue;
-                        const int base = B.bottom + 2 * (rr * e + cc);
-                        test_prim(S, o, d, 0, base, best);
-                        test_prim(S, o, d, 0, base + 1, best);
-                    }
+                const Hit keep = best;
+                const int base = B.bottom + 2 * (r * e + c);
+                test_prim(S, o, d, 0, base, best);
+                test_prim(S, o, d, 0, base + 1, best);
+                const bool found = best.prim != keep.prim || best.obj != keep.obj || best.t != keep.t;
+                if (!found)
+                    for (int dr = -1; dr <= 1; ++dr)
+                        for (int dc = -1; dc <= 1; ++dc) {
+                            const int rr = r + dr, cc = c + dc;
+                            if ((dr == 0 && dc == 0) || rr < 0 || rr >= e || cc < 0 || cc >= e) continue;
+                            const int nb = B.bottom + 2 * (rr * e + cc);
+                            test_prim(S, o, d, 0, nb, best);
+                            test_prim(S, o, d, 0, nb + 1, best);
+                        }
             }
         }
     }
-    // side walls x = +-1, y = +-1: the two triangles of the segment at the plane hit
+    // side walls x = +-1, y = +-1: the two triangles of the segment at the plane
+    // hit, when that hit lies within the wall's extent (neighbours on a miss)
+    auto wall = [&](float tp, float lat, float z, int base) {
+        if (!(tp >= 0.0f) || !(z >= -1e-5f) || !(z <= S.hmax + 1e-5f) || !(fabsf(lat) <= 1.0f + 1e-5f)) return;
+        const int k = clampi((int)floorf((lat + 1.0f) * S.inv_delta), 0, e - 1);
+        const Hit keep = best;
+        test_prim(S, o, d, 0, base + 2 * k, best);
+        test_prim(S, o, d, 0, base + 2 * k + 1, best);
+        if (best.prim == keep.prim && best.obj == keep.obj && best.t == keep.t)
+            for (int dk = -1; dk <= 1; dk += 2) {
+                const int kk = k + dk;
+                if (kk < 0 || kk >= e) continue;
+                test_prim(S, o, d, 0, base + 2 * kk, best);
+                test_prim(S, o, d, 0, base + 2 * kk + 1, best);
+            }
+    };
     if (d.x != 0.0f) {
         for (int side = 0; side < 2; ++side) {
             const float X = side ? 1.0f : -1.0f;
             const float tp = (X - o.x) / d.x;
-            if (tp >= 0.0f) {
-                const float y = fmaf(tp, d.y, o.y);
-                const int r = clampi((int)floorf((y + 1.0f) * S.inv_delta), 0, e - 1);
-                const int base = side ? B.right : B.left;
-                for (int dr = -1; dr <= 1; ++dr) {
-                    const int rr = r + dr;
-                    if (rr < 0 || rr >= e) continue;
-                    test_prim(S, o, d, 0, base + 2 * rr, best);
-                    test_prim(S, o, d, 0, base + 2 * rr + 1, best);
-                }
-            }
+            wall(tp, fmaf(tp, d.y, o.y), fmaf(tp, d.z, o.z), side ? B.right : B.left);
         }
     }
     if (d.y != 0.0f) {
         for (int side = 0; side < 2; ++side) {
             const float Y = side ? 1.0f : -1.0f;
             const float tp = (Y - o.y) / d.y;
-            if (tp >= 0.0f) {
-                const float x = fmaf(tp, d.x, o.x);
-                const int c = clampi((int)floorf((x + 1.0f) * S.inv_delta), 0, e - 1);
-                const int base = side ? B.back : B.front;
-                for (int dc = -1; dc <= 1; ++dc) {
-                    const int cc = c + dc;
-                    if (cc < 0 || cc >= e) continue;
-                    test_prim(S, o, d, 0, base + 2 * cc, best);
-                    test_prim(S, o, d, 0, base + 2 * cc + 1, best);
-                }
-            }
+            wall(tp, fmaf(tp, d.x, o.x), fmaf(tp, d.z, o.z), side ? B.back : B.front);
         }
     }
     // top surface: walk the cells under the ray's xy path while its z lies in

@@ -6,9 +6,10 @@
 // One thread per photon.  The path kernels keep a photon's whole path in
 // registers; the adjoint re-runs it, keeping a per-bounce tape (20 floats a
 // bounce, at most 8 bounces) in private memory, and sweeps it backwards.
-// Per-vertex gradients (Sobel normals, vertex heights) are accumulated with
-// device float atomics into [R, R] arrays, then one gather kernel applies the
-// transposed Sobel/normalisation to give dL/dh.
+// Per-vertex gradients (Sobel normals, vertex heights) gather per workgroup
+// in an LDS patch (a 16x16 tile of the ray grid lands on a few cells), are
+// flushed into [R, R] arrays with device float atomics, and one gather kernel
+// applies the transposed Sobel/normalisation to give dL/dh.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -312,13 +313,44 @@ __device__ __forceinline__ void transfer1_adj(V3 ng, V3 d, float t, V3 Du, V3 Dt
     gng = gng + (-gd_ / den) * (X - beta * d);
 }
 
-__global__ __launch_bounds__(256) void adjoint_kernel(SceneArgs S, const float* __restrict__ hmax, EmitArgs E,
-                                                      int64_t N, const float* __restrict__ gEp,
-                                                      const float* __restrict__ gxp, const float* __restrict__ gMp,
-                                                      float* __restrict__ grad_n, float* __restrict__ grad_z) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= N) return;
-    S.hmax = hmax[0];
+// Per-vertex gradient accumulator of one workgroup: a PxP patch of top
+// vertices in LDS (the rays of a 16x16 ray tile land on a few cells), global
+// atomics for vertices outside the patch.
+constexpr int kPatch = 8;
+
+struct PatchAcc {
+    float* n;  // [kPatch][kPatch][3] in LDS
+    float* z;  // [kPatch][kPatch] in LDS
+    int r0, c0, R;
+    float* grad_n;
+    float* grad_z;
+    __device__ __forceinline__ bool inside(int r, int c, int& k) const {
+        const int dr = r - r0, dc = c - c0;
+        k = dr * kPatch + dc;
+        return dr >= 0 && dr < kPatch && dc >= 0 && dc < kPatch;
+    }
+    __device__ __forceinline__ void add_n(int r, int c, V3 v) const {
+        int k;
+        if (inside(r, c, k)) {
+            atomicAdd(n + 3 * k, v.x);
+            atomicAdd(n + 3 * k + 1, v.y);
+            atomicAdd(n + 3 * k + 2, v.z);
+        } else {
+            atomic_add3(grad_n + 3 * (r * R + c), v);
+        }
+    }
+    __device__ __forceinline__ void add_z(int r, int c, float v) const {
+        int k;
+        if (inside(r, c, k))
+            atomicAdd(z + k, v);
+        else
+            unsafeAtomicAdd(grad_z + r * R + c, v);
+    }
+};
+
+__device__ __forceinline__ void adjoint_photon(const SceneArgs& S, const EmitArgs& E, int64_t i, int64_t N,
+                                               const float* __restrict__ gEp, const float* __restrict__ gxp,
+                                               const float* __restrict__ gMp, const PatchAcc& A) {
     int ch;
     PathState s = emit(E, i, ch);
     const float ei = E.ior[ch], eo = 1.0f;
@@ -449,8 +481,7 @@ __global__ __launch_bounds__(256) void adjoint_kernel(SceneArgs S, const float* 
             const V3 gm = (1.0f / mlen) * (gns - dot(gns, ns) * ns);
             const float w[3] = {w0, B.b1, B.b2};
             for (int v = 0; v < 3; ++v)
-                if (T.vr[v] >= 0 && B.prim < prim_base(S.R).bottom)
-                    atomic_add3(grad_n + 3 * (T.vr[v] * S.R + T.vc[v]), w[v] * gm);
+                if (T.vr[v] >= 0 && B.prim < prim_base(S.R).bottom) A.add_n(T.vr[v], T.vc[v], w[v] * gm);
         }
         // geometric normal -> vertex heights
         if (glen > 0.0f) {
@@ -459,13 +490,54 @@ __global__ __launch_bounds__(256) void adjoint_kernel(SceneArgs S, const float* 
             const V3 ge1 = cross(e2, gg), ge2 = cross(gg, e1);
             const float gz[3] = {-ge1.z - ge2.z, ge1.z, ge2.z};
             for (int v = 0; v < 3; ++v)
-                if (T.vr[v] >= 0) unsafeAtomicAdd(grad_z + T.vr[v] * S.R + T.vc[v], gz[v]);
+                if (T.vr[v] >= 0) A.add_z(T.vr[v], T.vc[v], gz[v]);
         }
         gd = gd_in;
         gDu = gDu_in;
         gDv = gDv_in;
         gDt = gDt_in;
         gDp = gDp_in;
+    }
+}
+
+// One workgroup per 16x16 tile of the emitter's ray grid (one channel):
+// gradients of the vertices the tile's rays hit gather in LDS first.
+__global__ __launch_bounds__(256) void adjoint_kernel(SceneArgs S, const float* __restrict__ hmax, EmitArgs E,
+                                                      int64_t N, int tiles_x, const float* __restrict__ gEp,
+                                                      const float* __restrict__ gxp, const float* __restrict__ gMp,
+                                                      float* __restrict__ grad_n, float* __restrict__ grad_z) {
+    __shared__ float pn[kPatch * kPatch * 3];
+    __shared__ float pz[kPatch * kPatch];
+    S.hmax = hmax[0];
+    const int ch = blockIdx.y;
+    const int tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
+    const int col = tx * 16 + (threadIdx.x & 15), row = ty * 16 + (threadIdx.x >> 4);
+    for (int k = threadIdx.x; k < kPatch * kPatch * 3; k += 256) pn[k] = 0.0f;
+    for (int k = threadIdx.x; k < kPatch * kPatch; k += 256) pz[k] = 0.0f;
+    PatchAcc A;
+    A.n = pn;
+    A.z = pz;
+    A.R = S.R;
+    A.grad_n = grad_n;
+    A.grad_z = grad_z;
+    // patch origin: one vertex before the cell under the tile's first ray centre
+    const float x0 = -1.0f + (float)(2 * tx * 16 + 1) / (float)E.n_side;
+    const float y0 = -1.0f + (float)(2 * ty * 16 + 1) / (float)E.n_side;
+    A.c0 = (int)floorf((x0 + 1.0f) * S.inv_delta) - 1;
+    A.r0 = (int)floorf((y0 + 1.0f) * S.inv_delta) - 1;
+    __syncthreads();
+    if (col < E.n_side && row < E.n_side) {
+        const int64_t i = ((int64_t)row * E.n_side + col) * E.channels + ch;
+        adjoint_photon(S, E, i, N, gEp, gxp, gMp, A);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < kPatch * kPatch; k += 256) {
+        const int r = A.r0 + k / kPatch, c = A.c0 + k % kPatch;
+        if (r < 0 || r >= S.R || c < 0 || c >= S.R) continue;  // never accumulated: outside the grid
+        const float vz = pz[k];
+        if (vz != 0.0f) unsafeAtomicAdd(grad_z + r * S.R + c, vz);
+        const V3 vn = v3(pn[3 * k], pn[3 * k + 1], pn[3 * k + 2]);
+        if (vn.x != 0.0f || vn.y != 0.0f || vn.z != 0.0f) atomic_add3(grad_n + 3 * (r * S.R + c), vn);
     }
 }
 
@@ -734,8 +806,10 @@ int hf_adjoint_f32(const float* h, const float* normals, const hf_scene* sc, con
     hipLaunchKernelGGL(zero_kernel, dim3(blocks_for(3 * RR)), dim3(256), 0, s, gn, 3 * RR);
     hipLaunchKernelGGL(zero_kernel, dim3(blocks_for(RR)), dim3(256), 0, s, gz, RR);
     hipLaunchKernelGGL(hmax_kernel, dim3(1), dim3(1024), 0, s, h, RR, hmax);
-    hipLaunchKernelGGL(adjoint_kernel, dim3(blocks_for(N)), dim3(256), 0, s, scene_args(h, normals, sc, hmax),
-                       (const float*)hmax, emit_args(sc, em), N, grad_Ep, grad_xp, grad_Mp, gn, gz);
+    const int tiles_x = (em->n_side + 15) / 16;
+    hipLaunchKernelGGL(adjoint_kernel, dim3((unsigned)(tiles_x * tiles_x), (unsigned)sc->channels), dim3(256), 0, s,
+                       scene_args(h, normals, sc, hmax), (const float*)hmax, emit_args(sc, em), N, tiles_x, grad_Ep,
+                       grad_xp, grad_Mp, gn, gz);
     hipLaunchKernelGGL(normals_adj_kernel, dim3(blocks_for(RR)), dim3(256), 0, s, h, R, s_sobel, (const float*)gn, gg);
     hipLaunchKernelGGL(grad_h_kernel, dim3(blocks_for(RR)), dim3(256), 0, s, R, (const float*)gg, (const float*)gz,
                        grad_h);
