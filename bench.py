@@ -195,6 +195,17 @@ def main():
 
     if args.workload == "landweber":
         return bench_landweber(args)
+    if args.workload == "copy":
+        # calibration for HBM byte counters: 5 device-to-device copies of 1 GiB
+        # (1048576 KiB read and written per copy)
+        import torch
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("[bench] copy calibration: 5 copies of 1 GiB")
+        return
 
     import numpy as np
     import torch

@@ -61,11 +61,15 @@ typedef enum pds_mode {
  * window (max_pixel_radius) is never altered. */
 typedef struct pds_tuning {
     int32_t tile;          /* square screen tile edge in pixels: 8, 16, 32 or 64 */
-    int32_t seg_photons;   /* photon records per tile segment (one workgroup each), <= 4096 */
+    int32_t seg_photons;   /* photon records per tile segment (one workgroup each), <= 1024 */
     int32_t block_photons; /* photons per binning workgroup */
     int32_t capacity_pct;  /* records reserved per 100 photons (default 200); more
                               records than that run the photon-order path */
-    int32_t reserved[4];
+    int32_t walk;          /* 0: walk each footprint's whole bounding box (default);
+                              1: walk only the ellipse's row chords (same results,
+                              bit for bit: the pixels left out contribute exactly
+                              zero; fewer pixels, more setup per row) */
+    int32_t reserved[3];
 } pds_tuning;
 
 /* Bytes of device workspace pds_forward_f32 needs for this problem in

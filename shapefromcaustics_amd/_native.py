@@ -27,7 +27,7 @@ class PdsTuning(ctypes.Structure):
     """Mirror of ``pds_tuning`` (include/pds.h)."""
 
     _fields_ = [("tile", ctypes.c_int32), ("seg_photons", ctypes.c_int32), ("block_photons", ctypes.c_int32),
-                ("capacity_pct", ctypes.c_int32), ("reserved", ctypes.c_int32 * 4)]
+                ("capacity_pct", ctypes.c_int32), ("walk", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
 
 
 class PdsBinsStats(ctypes.Structure):

@@ -31,6 +31,7 @@
 // a device flag written by plan chooses, so the host never synchronises.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "pds_binned.hpp"
@@ -43,7 +44,7 @@ constexpr int kBinThreads = 512;
 constexpr int kMaxBins = 16384;
 constexpr int kMaxBinBlocks = 1024;
 constexpr int kTileThreads = 256;
-constexpr int kMaxSeg = 4096;  // records per segment (LDS sort index is 16-bit)
+constexpr int kMaxSeg = 1024;  // records per segment (sizes the LDS sort of a segment)
 constexpr int kMinBins = 256;
 constexpr int kDefaultSeg = 1024;
 constexpr double kDefaultCapacity = 2.0;  // records per photon reserved in the workspace
@@ -64,6 +65,7 @@ struct Geom {
     int32_t T, tshift, tiles_x, tiles_y, nbins, G, seg;
     int64_t per_block, cap_records, max_segments;
     bool det;
+    bool box_walk;  // walk whole footprint boxes (default) instead of the ellipse's row chords
 };
 
 struct Layout {
@@ -120,6 +122,7 @@ bool make_geom(int64_t N, int32_t C, int32_t H, int32_t W, int32_t cap, int32_t 
     if (g.cap_records >= ((int64_t)1 << 32)) return false;  // 32-bit record offsets
     g.max_segments = g.nbins + g.cap_records / g.seg + 1;
     g.det = mode == PDS_MODE_DETERMINISTIC;
+    g.box_walk = !(t && t->walk == 1);
     return true;
 }
 
@@ -474,17 +477,106 @@ __device__ __forceinline__ float from_fixed(unsigned long long v, int e) {
 }
 
 // One record decoded for the tile with origin (X0, Y0) and last pixel (X1, Y1):
-// its footprint box = window n tile n ellipse bounding box.
+// its footprint box = window n tile n ellipse bounding box, and the chord
+// parameters of its rows (see row_span).
 struct TileRec {
     Frame f;            // ix, iy absolute
     float e;
     int32_t lx0, ly0;   // box origin in tile coordinates
     int32_t kx0, ky0;   // box origin relative to (ix, iy)
     int32_t bw, area;   // box width and area (0: empty)
+    int32_t rows;       // box height
+    float A, invA, det, P;  // q = A dx^2 + 2 P dy dx + Q dy^2, det = a00 a11 - a01 a10
+    bool full;          // no usable chords: walk whole box rows
+};
+
+// The pixels of box row ky (offset from iy) that can have q < 1, as a column
+// range [xa, xb] (offsets from ix) within the box; xa > xb when none.  For
+// fixed dy, q is a quadratic in dx with minimum (det dy)^2 / A at
+// dx* = -P dy / A (Lagrange's identity), so the chord is dx* +- w with
+// w = sqrt(A - (det dy)^2) / A.  The chord is widened (1e-5 A under the root,
+// 1e-3 px + 1e-4 relative outside) far beyond the float error of q near 1, so
+// every pixel the exact evaluation would keep is inside; pixels left out have
+// q >= 1 and contribute exactly nothing, and the walk over the chords visits
+// the kept pixels in the same order as the walk over the whole box.
+__device__ __forceinline__ void row_span(const TileRec& t, int ky, int& xa, int& xb) {
+    const int bx1 = t.kx0 + t.bw - 1;
+    if (t.full) {
+        xa = t.kx0;
+        xb = bx1;
+        return;
+    }
+    // explicitly rounded operations: the sort and the walk evaluate this in
+    // separate inlined copies and must get the same chord
+    const float dy = __fsub_rn((float)ky, t.f.fy);
+    const float dd = __fmul_rn(t.det, dy);
+    const float rad = fmaf(-dd, dd, t.A);
+    const float eps = __fmul_rn(1e-5f, t.A);
+    if (!(rad > -eps)) {
+        xa = 1;
+        xb = 0;
+        return;
+    }
+    const float w = __fmul_rn(__builtin_amdgcn_sqrtf(__fadd_rn(fmaxf(rad, 0.0f), eps)), t.invA);
+    const float c = fmaf(__fmul_rn(-t.P, dy), t.invA, t.f.fx);
+    const float m = fmaf(1e-4f, __fadd_rn(fabsf(c), w), 1e-3f);
+    const float lo = __fsub_rn(__fsub_rn(c, w), m), hi = __fadd_rn(__fadd_rn(c, w), m);
+    // clamp in float before converting (c, w may be huge for a thin footprint)
+    xa = (int)ceilf(fminf(fmaxf(lo, (float)t.kx0), (float)(bx1 + 1)));
+    xb = (int)floorf(fmaxf(fminf(hi, (float)bx1), (float)(t.kx0 - 1)));
+}
+
+// Pixels of the record's chords (the walk's length); exact box area when full.
+__device__ __forceinline__ int chord_pixels(const TileRec& t) {
+    if (t.area == 0) return 0;
+    if (t.full) return t.area;
+    int n = 0;
+    for (int r = 0; r < t.rows; ++r) {
+        int xa, xb;
+        row_span(t, t.ky0 + r, xa, xb);
+        n += max(xb - xa + 1, 0);
+    }
+    return n;
+}
+
+// The walk over a record's chords, one pixel per step: current row r, column
+// kx (offset from ix), end of the row's chord xb, LDS index li, row offset dy.
+struct ChordWalk {
+    int r, kx, xb, li;
+    float dy;
+    // move to the first pixel of the first non-empty chord at or after row r
+    __device__ __forceinline__ void seek(const TileRec& t, int tshift) {
+        int xa = 1;
+        xb = 0;
+        for (; r < t.rows; ++r) {
+            row_span(t, t.ky0 + r, xa, xb);
+            if (xa <= xb) break;
+        }
+        if (r >= t.rows) {  // not reached when the walk's length is the chord count: stay inside the box
+            r = t.rows - 1;
+            xa = xb = t.kx0;
+        }
+        kx = xa;
+        li = ((t.ly0 + r) << tshift) + t.lx0 + (xa - t.kx0);
+        dy = (float)(t.ky0 + r) - t.f.fy;
+    }
+    __device__ __forceinline__ void start(const TileRec& t, int tshift) {
+        r = 0;
+        seek(t, tshift);
+    }
+    // after a pixel: next column, or the next non-empty chord
+    __device__ __forceinline__ void step(const TileRec& t, int tshift, bool more) {
+        ++kx;
+        ++li;
+        if (kx > xb && more) {
+            ++r;
+            seek(t, tshift);
+        }
+    }
 };
 
 __device__ __forceinline__ TileRec decode_tile_rec(const uint4* __restrict__ recs, size_t j, int X0, int Y0, int X1,
-                                                   int Y1, int half) {
+                                                   int Y1, int half, bool box_walk) {
     TileRec t;
     const uint4 q0 = recs[2 * j], q1 = recs[2 * j + 1];
     const uint32_t pk = q0.w;
@@ -506,17 +598,25 @@ __device__ __forceinline__ TileRec decode_tile_rec(const uint4* __restrict__ rec
     clip_to_ellipse(t.f, x0, x1, y0, y1);
     const bool any = x0 <= x1 && y0 <= y1;
     t.bw = any ? x1 - x0 + 1 : 1;
-    t.area = any ? t.bw * (y1 - y0 + 1) : 0;
+    t.rows = any ? y1 - y0 + 1 : 0;
+    t.area = t.bw * t.rows;
     t.lx0 = x0 - X0;
     t.ly0 = y0 - Y0;
     t.kx0 = x0 - ix;
     t.ky0 = y0 - iy;
+    const Frame& f = t.f;
+    t.A = fmaf(f.a00, f.a00, f.a10 * f.a10);
+    t.P = fmaf(f.a00, f.a01, f.a10 * f.a11);
+    t.det = det2(f.a00, f.a01, f.a10, f.a11);
+    t.invA = 1.0f / t.A;
+    t.full = !(t.A > 1e-30f) || !(t.A < 1e30f) || !(fabsf(t.det) < INFINITY) || !(fabsf(t.P) < INFINITY) ||
+             !(t.invA < INFINITY) || box_walk;
     return t;
 }
 
-// Counting sort of a segment's records by footprint area (clamped to 255), so
-// that the 64 records a wavefront walks side by side, one per lane, have
-// nearly the same number of pixels.
+// Counting sort of a segment's records by the pixels of their chords (clamped
+// to 255; the key is the exact count below that), so that the 64 records a
+// wavefront walks side by side, one per lane, have nearly the same length.
 struct SegSort {
     uint16_t order[kMaxSeg];
     uint8_t key[kMaxSeg];
@@ -525,14 +625,14 @@ struct SegSort {
 };
 
 __device__ __forceinline__ void sort_segment(SegSort& S, const uint4* __restrict__ recs, const int4& d, int X0, int Y0,
-                                             int X1, int Y1, int half) {
+                                             int X1, int Y1, int half, bool box_walk) {
     static_assert(kTileThreads == 256, "one histogram bin per thread");
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     S.hist[tid] = 0;
     __syncthreads();
     for (int j = tid; j < d.z; j += kTileThreads) {
-        const TileRec t = decode_tile_rec(recs, (size_t)d.y + j, X0, Y0, X1, Y1, half);
-        const uint32_t key = (uint32_t)min(t.area, 255);
+        const TileRec t = decode_tile_rec(recs, (size_t)d.y + j, X0, Y0, X1, Y1, half, box_walk);
+        const uint32_t key = (uint32_t)min(chord_pixels(t), 255);
         S.key[j] = (uint8_t)key;
         atomicAdd(&S.hist[key], 1u);
     }
@@ -613,10 +713,6 @@ __device__ __forceinline__ void flat_splat_all(const PhotonArgs& P, const ImageA
 //    records (area-sorted) into the LDS tile, which is then flushed.  On
 //    overflow the workgroups run the photon-order walk instead.
 // ---------------------------------------------------------------------------
-union TileShared {
-    SegSort sort;
-    WaveRecs flat[kTileThreads / 64];
-};
 
 template <bool DET>
 __global__ __launch_bounds__(kTileThreads) void bin_tile_kernel(Geom g, PhotonArgs P, ImageArgs I,
@@ -626,11 +722,11 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_kernel(Geom g, PhotonAr
                                                                 unsigned long long* __restrict__ acc64) {
     using Acc = typename std::conditional<DET, unsigned long long, float>::type;
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ TileShared U;
-    SegSort& S = U.sort;
+    __shared__ SegSort S;
     Acc* tile = reinterpret_cast<Acc*>(smem);
     if (ctrl->overflow) {
-        flat_splat_all<DET>(P, I, DET ? ctrl->fx_exp : 0, U.flat, out, acc64);
+        // the dynamic LDS (sized for both) holds the walk's per-wave buffers
+        flat_splat_all<DET>(P, I, DET ? ctrl->fx_exp : 0, reinterpret_cast<WaveRecs*>(smem), out, acc64);
         return;
     }
     const uint32_t s = blockIdx.x;
@@ -641,33 +737,29 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_kernel(Geom g, PhotonAr
     for (int k = threadIdx.x; k < TT; k += kTileThreads) tile[k] = Acc(0);
     const SegTile st = seg_tile(g, I, d.x);
     const int half = T >> 1;
-    sort_segment(S, recs, d, st.X0, st.Y0, st.X1, st.Y1, half);
+    sort_segment(S, recs, d, st.X0, st.Y0, st.X1, st.Y1, half, g.box_walk);
     for (int pos = threadIdx.x; pos < d.z; pos += kTileThreads) {
-        const TileRec t = decode_tile_rec(recs, (size_t)d.y + S.order[pos], st.X0, st.Y0, st.X1, st.Y1, half);
-        int row = 0, col = 0;
-        int idx = (t.ly0 << g.tshift) + t.lx0;
-        float dy = (float)t.ky0 - t.f.fy;
-        for (int p = 0; p < t.area; ++p, ++idx) {
-            const float dx = (float)(t.kx0 + col) - t.f.fx;
+        const int j = S.order[pos];
+        const TileRec t = decode_tile_rec(recs, (size_t)d.y + j, st.X0, st.Y0, st.X1, st.Y1, half, g.box_walk);
+        const int n = S.key[j] < 255 ? (int)S.key[j] : chord_pixels(t);
+        ChordWalk w;
+        if (n > 0) w.start(t, g.tshift);
+        for (int p = 0; p < n; ++p) {
+            const float dx = (float)w.kx - t.f.fx;
             float u, v;
-            const float val = silverman(eval_q(t.f, dx, dy, u, v)) * t.e;
+            const float val = silverman(eval_q(t.f, dx, w.dy, u, v)) * t.e;
             if (val > 0.0f) {
                 if constexpr (DET) {
                     const unsigned long long fxv = to_fixed(val, fe);
                     if (fxv >> 62)
-                        atomicOr(&tile[idx], fxv);
+                        atomicOr(&tile[w.li], fxv);
                     else
-                        atomicAdd(&tile[idx], fxv);
+                        atomicAdd(&tile[w.li], fxv);
                 } else {
-                    atomicAdd(&tile[idx], val);
+                    atomicAdd(&tile[w.li], val);
                 }
             }
-            if (++col == t.bw) {
-                col = 0;
-                ++row;
-                idx += T - t.bw;
-                dy = (float)(t.ky0 + row) - t.f.fy;
-            }
+            w.step(t, g.tshift, p + 1 < n);
         }
     }
     __syncthreads();
@@ -747,11 +839,12 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_bwd_kernel(Geom g, Imag
         gt[k] = (px < I.W && py < I.H) ? grad[((int64_t)st.c * I.H + py) * I.W + px] : 0.0f;
     }
     const int half = T >> 1;
-    sort_segment(S, recs, d, st.X0, st.Y0, st.X1, st.Y1, half);
+    sort_segment(S, recs, d, st.X0, st.Y0, st.X1, st.Y1, half, g.box_walk);
     const float hw = 0.5f * I.fw, hh = 0.5f * I.fh, iw = 2.0f / I.fw, ih = 2.0f / I.fh;
     for (int pos = threadIdx.x; pos < d.z; pos += kTileThreads) {
         const int j = S.order[pos];
-        const TileRec t = decode_tile_rec(recs, (size_t)d.y + j, st.X0, st.Y0, st.X1, st.Y1, half);
+        const TileRec t = decode_tile_rec(recs, (size_t)d.y + j, st.X0, st.Y0, st.X1, st.Y1, half, g.box_walk);
+        const int n = S.key[j] < 255 ? (int)S.key[j] : chord_pixels(t);
         // Per pixel, with w = (1 - q) inside the footprint and 0 outside,
         // accumulate  m0 = sum w^2 g  and the moments of  w g  over the pixel
         // offsets (dx, dy):  m1 = sum w g dx, m2 = sum w g dy, m3 = sum w g dx^2,
@@ -759,15 +852,14 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_bwd_kernel(Geom g, Imag
         // c = -(12/pi) E, and (u, v) = A (dx, dy), the reference's sums
         // (photon_differentials.cu:195-204) are linear in these moments.
         float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f, m4 = 0.f, m5 = 0.f;
-        int col = 0, row = 0;
-        int li = (t.ly0 << g.tshift) + t.lx0;
-        float dy = (float)t.ky0 - t.f.fy;
-        for (int p = 0; p < t.area; ++p) {
-            const float dx = (float)(t.kx0 + col) - t.f.fx;
+        ChordWalk cw;
+        if (n > 0) cw.start(t, g.tshift);
+        for (int p = 0; p < n; ++p) {
+            const float dx = (float)cw.kx - t.f.fx, dy = cw.dy;
             float u, v;
             const float q = eval_q(t.f, dx, dy, u, v);
             const float w = q < 1.0f ? 1.0f - q : 0.0f;
-            const float wg = w * gt[li];
+            const float wg = w * gt[cw.li];
             m0 = fmaf(w, wg, m0);
             const float px = wg * dx, py = wg * dy;
             m1 += px;
@@ -775,13 +867,7 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_bwd_kernel(Geom g, Imag
             m3 = fmaf(px, dx, m3);
             m4 = fmaf(px, dy, m4);
             m5 = fmaf(py, dy, m5);
-            ++li;
-            if (++col == t.bw) {
-                col = 0;
-                ++row;
-                li += T - t.bw;
-                dy = (float)(t.ky0 + row) - t.f.fy;
-            }
+            cw.step(t, g.tshift, p + 1 < n);
         }
         const Frame& f = t.f;
         const float c = -kTwelveOverPi * t.e;
@@ -1016,7 +1102,8 @@ void launch_build(const PhotonArgs& P, const ImageArgs& I, const Geom& g, const 
 void launch_forward_tiles(const PhotonArgs& P, const ImageArgs& I, const Geom& g, const Buffers& B, float* out,
                           hipStream_t s) {
     const unsigned nsegs = (unsigned)g.max_segments;
-    const size_t tile_lds = (size_t)g.T * g.T * (g.det ? 8 : 4);
+    // the tile accumulator, or (photon-order path) four waves' walk buffers
+    const size_t tile_lds = std::max((size_t)g.T * g.T * (g.det ? 8 : 4), sizeof(WaveRecs) * (kTileThreads / 64));
     if (g.det) {
         hipLaunchKernelGGL(bin_tile_kernel<true>, dim3(nsegs), dim3(kTileThreads), tile_lds, s, g, P, I,
                            (const int4*)B.segs, (const uint4*)B.recs, (const Ctrl*)B.ctrl, out, B.acc64);

@@ -296,3 +296,41 @@ def test_segment_sizes(seg):
     img, grads = _run(p, "auto", g, tuning={"seg_photons": seg, "tile": 16})
     assert_image_close(img, O.forward_f64(*p.arrays(), p.size, p.cap), f"seg {seg} fwd")
     assert_grads_close(grads, O.backward_f64(g, *p.arrays(), p.cap), f"seg {seg} bwd")
+
+
+# ----------------------------------------------------------------------------- chord walk
+
+
+def _thin_photons(n, size, cap, seed):
+    """Extreme footprints: axes logU(0.01, 20) px, any rotation, so chords are
+    often empty or a single pixel and boxes far larger than the ellipse."""
+    rng = np.random.default_rng(seed)
+    C, H, W = size
+    px, py = rng.uniform(-3, W + 3, n), rng.uniform(-3, H + 3, n)
+    theta = rng.uniform(0, np.pi, n)
+    s1 = np.exp(rng.uniform(np.log(0.01), np.log(20.0), n))
+    s2 = np.exp(rng.uniform(np.log(0.01), np.log(20.0), n))
+    E = rng.uniform(0.5, 1.5, n) / n
+    return WL.from_pixel_space(px, py, theta, s1, s2, E, np.arange(n) % C, size, cap)
+
+
+@pytest.mark.parametrize("case", ["caustic", "thin"])
+@pytest.mark.parametrize("tile", [16, 64])
+def test_chord_walk_equals_box_walk(case, tile):
+    # walking only each footprint's row chords leaves out pixels of the
+    # bounding box that have q >= 1, so the results must be bit-identical to
+    # walking the whole box (deterministic forward, every backward)
+    p = (WL.caustic(30000, (3, 128, 160), 12, seed=5) if case == "caustic"
+         else _thin_photons(30000, (2, 120, 136), 20, seed=6))
+    g = torch.from_numpy(WL.upstream_grad(p.size, seed=8)).cuda()
+    E, xp, M, cp, radius = _dev(p)
+    outs = {}
+    for walk in (0, 1):
+        tun = {"tile": tile, "seg_photons": 256, "walk": walk}
+        img = pds_forward(E, xp, M, cp, radius, p.size, p.cap, mode="deterministic", tuning=tun)[0]
+        grads = pds_backward(g, E, xp, M, cp, radius, p.cap, mode="binned", tuning=tun)
+        outs[walk] = (img, grads)
+    assert torch.equal(outs[0][0], outs[1][0]), "deterministic image differs between box and chord walks"
+    for a, b in zip(outs[0][1], outs[1][1]):
+        assert torch.equal(a, b), "gradients differ between chord and box walks"
+    assert_image_close(outs[1][0], O.forward_f64(*p.arrays(), p.size, p.cap), f"{case} chord walk")
