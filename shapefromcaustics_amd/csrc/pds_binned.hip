@@ -111,7 +111,7 @@ bool make_geom(int64_t N, int32_t C, int32_t H, int32_t W, int32_t cap, int32_t 
     g.seg = (t && t->seg_photons > 0) ? t->seg_photons : kDefaultSeg;
     if (g.seg > kMaxSeg) g.seg = kMaxSeg;
     const int64_t bp = (t && t->block_photons > 0) ? t->block_photons : 0;
-    int64_t G = bp > 0 ? (N + bp - 1) / bp : (N + 4095) / 4096;
+    int64_t G = bp > 0 ? (N + bp - 1) / bp : (N + 8191) / 8192;
     if (G > kMaxBinBlocks) G = kMaxBinBlocks;
     if (G < 1) G = 1;
     g.G = (int32_t)G;
@@ -181,24 +181,47 @@ struct BinPhoton {
 // "special" and the backward evaluates them by a direct serial walk.  Photons
 // with E <= 0 get records (their gE is sum K g); the forward's val > 0 test
 // (:106) skips their pixels.
-__device__ __forceinline__ BinPhoton bin_photon(const PhotonArgs& P, const ImageArgs& I, int64_t i) {
+// The inputs of one photon, loaded together so that the next photon's loads
+// can be in flight while the current one is binned.
+struct RawPhoton {
+    int64_t c;
+    float e, cx, cy, radius, m00, m01, m02, m10, m11, m12;
+};
+
+__device__ __forceinline__ RawPhoton load_photon(const PhotonArgs& P, int64_t i) {
+    const int64_t N = P.N;
+    RawPhoton r;
+    r.c = P.cp[i];
+    r.e = P.E[i];
+    r.cx = P.xp[i];
+    r.cy = P.xp[N + i];
+    r.radius = P.radius[i];
+    r.m00 = P.M[i];
+    r.m01 = P.M[N + i];
+    r.m02 = P.M[2 * N + i];
+    r.m10 = P.M[3 * N + i];
+    r.m11 = P.M[4 * N + i];
+    r.m12 = P.M[5 * N + i];
+    return r;
+}
+
+__device__ __forceinline__ BinPhoton bin_photon(const RawPhoton& R, const ImageArgs& I) {
     BinPhoton b;
     b.rec = false;
     b.special = false;
-    const int64_t c = P.cp[i];
+    const int64_t c = R.c;
     if (c < 0 || c >= I.C) return b;
-    const int64_t N = P.N;
-    const float e = P.E[i];
-    const float cx = P.xp[i], cy = P.xp[N + i];
-    const Window win = photon_window(P.radius[i], cx, cy, I.W, I.H, I.cap);
+    const float e = R.e;
+    const float cx = R.cx, cy = R.cy;
+    const Window win = photon_window(R.radius, cx, cy, I.W, I.H, I.cap);
     if (win.rx < 0 || win.ry < 0) return b;
     int32_t x0 = (int32_t)max((int64_t)win.pxc - win.rx, (int64_t)0);
     int32_t x1 = (int32_t)min((int64_t)win.pxc + win.rx, (int64_t)I.W - 1);
     int32_t y0 = (int32_t)max((int64_t)win.pyc - win.ry, (int64_t)0);
     int32_t y1 = (int32_t)min((int64_t)win.pyc + win.ry, (int64_t)I.H - 1);
     if (x0 > x1 || y0 > y1) return b;
-    const float m00 = P.M[i], m01 = P.M[N + i], m10 = P.M[3 * N + i], m11 = P.M[4 * N + i];
-    const float poison = P.M[2 * N + i] * 0.0f + P.M[5 * N + i] * 0.0f;
+    const float m00 = R.m00, m01 = R.m01, m10 = R.m10, m11 = R.m11;
+    const float poison = R.m02 * 0.0f + R.m12 * 0.0f;
     const bool finite_frame = isfinite(cx) && isfinite(cy) && isfinite(m00) && isfinite(m01) && isfinite(m10) &&
                               isfinite(m11) && poison == 0.0f;
     b.special = !finite_frame || !isfinite(e);
@@ -264,8 +287,13 @@ __global__ __launch_bounds__(kBinThreads) void bin_count_kernel(PhotonArgs P, Im
     const int64_t lo = (int64_t)blockIdx.x * g.per_block;
     const int64_t hi = min(lo + g.per_block, P.N);
     float emax = 0.0f;
-    for (int64_t i = lo + threadIdx.x; i < hi; i += kBinThreads) {
-        const BinPhoton b = bin_photon(P, I, i);
+    int64_t i = lo + threadIdx.x;
+    RawPhoton raw;
+    if (i < hi) raw = load_photon(P, i);
+    for (; i < hi; i += kBinThreads) {
+        const RawPhoton now = raw;
+        if (i + kBinThreads < hi) raw = load_photon(P, i + kBinThreads);  // prefetch
+        const BinPhoton b = bin_photon(now, I);
         if (!b.rec) continue;
         if (b.e > 0.0f && isfinite(b.e)) emax = fmaxf(emax, b.e);
         const int tx0 = b.x0 >> g.tshift, tx1 = b.x1 >> g.tshift;
@@ -415,8 +443,13 @@ __global__ __launch_bounds__(kBinThreads) void bin_scatter_kernel(PhotonArgs P, 
     const int64_t lo = (int64_t)blockIdx.x * g.per_block;
     const int64_t hi = min(lo + g.per_block, P.N);
     const int half = g.T >> 1;
-    for (int64_t i = lo + threadIdx.x; i < hi; i += kBinThreads) {
-        const BinPhoton b = bin_photon(P, I, i);
+    int64_t i = lo + threadIdx.x;
+    RawPhoton raw;
+    if (i < hi) raw = load_photon(P, i);
+    for (; i < hi; i += kBinThreads) {
+        const RawPhoton now = raw;
+        if (i + kBinThreads < hi) raw = load_photon(P, i + kBinThreads);  // prefetch
+        const BinPhoton b = bin_photon(now, I);
         int n = 0;
         if (b.rec) {
             const uint4 r0 = make_uint4(__float_as_uint(b.f.fx), __float_as_uint(b.f.fy), __float_as_uint(b.e), 0u);
