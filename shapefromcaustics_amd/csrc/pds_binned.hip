@@ -69,7 +69,7 @@ struct Geom {
 };
 
 struct Layout {
-    size_t hist, totals, binoff, emax, segs, ctrl, recs, slots, info, partials, acc64, end;
+    size_t hist, totals, binoff, emax, segs, ctrl, recs, slots, info, partials, order, acc64, end;
 };
 
 inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -145,13 +145,15 @@ Layout make_layout(const Geom& g, int64_t N, int32_t C, int32_t H, int32_t W, bo
     o = align256(o + sizeof(Ctrl));
     L.recs = o;
     o = align256(o + 32 * (size_t)g.cap_records);
-    L.slots = L.info = L.partials = o;
+    L.slots = L.info = L.partials = L.order = o;
     if (keep) {
         o = align256(o + sizeof(uint32_t) * kMaxRecsPerPhoton * (size_t)N);
         L.info = o;
         o = align256(o + (size_t)N);
         L.partials = o;
         o = align256(o + 32 * (size_t)g.cap_records);
+        L.order = o;  // the forward's walk order of every segment, for the backward
+        o = align256(o + sizeof(uint32_t) * (size_t)g.cap_records);
     }
     L.acc64 = o;
     if (g.det) o = align256(o + sizeof(uint64_t) * (size_t)C * H * W);
@@ -752,7 +754,8 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_kernel(Geom g, PhotonAr
                                                                 const int4* __restrict__ segs,
                                                                 const uint4* __restrict__ recs, const Ctrl* ctrl,
                                                                 float* __restrict__ out,
-                                                                unsigned long long* __restrict__ acc64) {
+                                                                unsigned long long* __restrict__ acc64,
+                                                                uint32_t* __restrict__ order_out) {
     using Acc = typename std::conditional<DET, unsigned long long, float>::type;
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ SegSort S;
@@ -771,6 +774,11 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_kernel(Geom g, PhotonAr
     const SegTile st = seg_tile(g, I, d.x);
     const int half = T >> 1;
     sort_segment(S, recs, d, st.X0, st.Y0, st.X1, st.Y1, half, g.box_walk);
+    if (order_out)  // keep the walk order (record | key << 16) for the backward
+        for (int pos = threadIdx.x; pos < d.z; pos += kTileThreads) {
+            const uint32_t j = S.order[pos];
+            order_out[(size_t)d.y + pos] = j | ((uint32_t)S.key[j] << 16);
+        }
     for (int pos = threadIdx.x; pos < d.z; pos += kTileThreads) {
         const int j = S.order[pos];
         const TileRec t = decode_tile_rec(recs, (size_t)d.y + j, st.X0, st.Y0, st.X1, st.Y1, half, g.box_walk);
@@ -856,12 +864,15 @@ __global__ __launch_bounds__(kTileThreads) void bin_convert_split_kernel(Geom g,
 // lane walks whole records (area-sorted), sums their seven terms in
 // registers and stores them, in the reference's units, at the record's slot.
 // ---------------------------------------------------------------------------
+// SORTED: the forward of the same binning left each segment's walk order in
+// order_in, so the segment is not sorted again.
+template <bool SORTED>
 __global__ __launch_bounds__(kTileThreads) void bin_tile_bwd_kernel(Geom g, ImageArgs I, const int4* __restrict__ segs,
                                                                     const uint4* __restrict__ recs, const Ctrl* ctrl,
                                                                     const float* __restrict__ grad,
-                                                                    float4* __restrict__ partials) {
+                                                                    float4* __restrict__ partials,
+                                                                    const uint32_t* __restrict__ order_in) {
     extern __shared__ __align__(16) float gt[];  // T * T upstream gradient tile
-    __shared__ SegSort S;
     const uint32_t s = blockIdx.x;
     if (s >= ctrl->nseg) return;
     const int4 d = segs[s];
@@ -872,12 +883,29 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_bwd_kernel(Geom g, Imag
         gt[k] = (px < I.W && py < I.H) ? grad[((int64_t)st.c * I.H + py) * I.W + px] : 0.0f;
     }
     const int half = T >> 1;
-    sort_segment(S, recs, d, st.X0, st.Y0, st.X1, st.Y1, half, g.box_walk);
+    const uint16_t* order = nullptr;
+    const uint8_t* keys = nullptr;
+    if constexpr (!SORTED) {
+        __shared__ SegSort S;
+        sort_segment(S, recs, d, st.X0, st.Y0, st.X1, st.Y1, half, g.box_walk);
+        order = S.order;
+        keys = S.key;
+    } else {
+        __syncthreads();
+    }
     const float hw = 0.5f * I.fw, hh = 0.5f * I.fh, iw = 2.0f / I.fw, ih = 2.0f / I.fh;
     for (int pos = threadIdx.x; pos < d.z; pos += kTileThreads) {
-        const int j = S.order[pos];
+        int j, key;
+        if constexpr (SORTED) {
+            const uint32_t w = order_in[(size_t)d.y + pos];
+            j = (int)(w & 0xffffu);
+            key = (int)(w >> 16);
+        } else {
+            j = order[pos];
+            key = keys[j];
+        }
         const TileRec t = decode_tile_rec(recs, (size_t)d.y + j, st.X0, st.Y0, st.X1, st.Y1, half, g.box_walk);
-        const int n = S.key[j] < 255 ? (int)S.key[j] : chord_pixels(t);
+        const int n = key < 255 ? key : chord_pixels(t);
         // Per pixel, with w = (1 - q) inside the footprint and 0 outside,
         // accumulate  m0 = sum w^2 g  and the moments of  w g  over the pixel
         // offsets (dx, dy):  m1 = sum w g dx, m2 = sum w g dy, m3 = sum w g dx^2,
@@ -1086,6 +1114,7 @@ struct Buffers {
     uint4* recs;
     uint8_t* info;
     float4* partials;
+    uint32_t* order;
     unsigned long long* acc64;
 };
 
@@ -1102,6 +1131,7 @@ Buffers buffers(void* ws, const Layout& L) {
     B.slots = (uint32_t*)(b + L.slots);
     B.info = (uint8_t*)(b + L.info);
     B.partials = (float4*)(b + L.partials);
+    B.order = (uint32_t*)(b + L.order);
     B.acc64 = (unsigned long long*)(b + L.acc64);
     return B;
 }
@@ -1133,28 +1163,36 @@ void launch_build(const PhotonArgs& P, const ImageArgs& I, const Geom& g, const 
 
 // tiles (-> convert); the same launches run the photon-order path on overflow
 void launch_forward_tiles(const PhotonArgs& P, const ImageArgs& I, const Geom& g, const Buffers& B, float* out,
-                          hipStream_t s) {
+                          bool keep, hipStream_t s) {
     const unsigned nsegs = (unsigned)g.max_segments;
     // the tile accumulator, or (photon-order path) four waves' walk buffers
     const size_t tile_lds = std::max((size_t)g.T * g.T * (g.det ? 8 : 4), sizeof(WaveRecs) * (kTileThreads / 64));
     if (g.det) {
         hipLaunchKernelGGL(bin_tile_kernel<true>, dim3(nsegs), dim3(kTileThreads), tile_lds, s, g, P, I,
-                           (const int4*)B.segs, (const uint4*)B.recs, (const Ctrl*)B.ctrl, out, B.acc64);
+                           (const int4*)B.segs, (const uint4*)B.recs, (const Ctrl*)B.ctrl, out, B.acc64,
+                           keep ? B.order : (uint32_t*)nullptr);
         hipLaunchKernelGGL(bin_convert_split_kernel, dim3(nsegs), dim3(kTileThreads), 0, s, g, I,
                            (const int4*)B.segs, (const Ctrl*)B.ctrl, (const unsigned long long*)B.acc64, out);
     } else {
         hipLaunchKernelGGL(bin_tile_kernel<false>, dim3(nsegs), dim3(kTileThreads), tile_lds, s, g, P, I,
-                           (const int4*)B.segs, (const uint4*)B.recs, (const Ctrl*)B.ctrl, out, B.acc64);
+                           (const int4*)B.segs, (const uint4*)B.recs, (const Ctrl*)B.ctrl, out, B.acc64,
+                           keep ? B.order : (uint32_t*)nullptr);
     }
 }
 
 // tile adjoint -> finalize -> direct walk of special photons (of every photon on overflow)
 void launch_backward_tiles(const float* grad, const PhotonArgs& P, const ImageArgs& I, const Geom& g,
-                           const Buffers& B, const GradArgs& G, hipStream_t s) {
+                           const Buffers& B, const GradArgs& G, bool sorted, hipStream_t s) {
     const unsigned nsegs = (unsigned)g.max_segments;
     const size_t tile_lds = (size_t)g.T * g.T * 4;
-    hipLaunchKernelGGL(bin_tile_bwd_kernel, dim3(nsegs), dim3(kTileThreads), tile_lds, s, g, I, (const int4*)B.segs,
-                       (const uint4*)B.recs, (const Ctrl*)B.ctrl, grad, B.partials);
+    if (sorted)
+        hipLaunchKernelGGL(bin_tile_bwd_kernel<true>, dim3(nsegs), dim3(kTileThreads), tile_lds, s, g, I,
+                           (const int4*)B.segs, (const uint4*)B.recs, (const Ctrl*)B.ctrl, grad, B.partials,
+                           (const uint32_t*)B.order);
+    else
+        hipLaunchKernelGGL(bin_tile_bwd_kernel<false>, dim3(nsegs), dim3(kTileThreads), tile_lds, s, g, I,
+                           (const int4*)B.segs, (const uint4*)B.recs, (const Ctrl*)B.ctrl, grad, B.partials,
+                           (const uint32_t*)nullptr);
     int64_t blocks = (P.N + 255) / 256;
     if (blocks > 8192) blocks = 8192;
     if (blocks < 1) blocks = 1;
@@ -1228,7 +1266,7 @@ int binned_forward(const PhotonArgs& P, const ImageArgs& I, int32_t mode, const 
     }
     const Buffers B = buffers(ws, L);
     launch_build(P, I, g, B, keep, out, s);
-    launch_forward_tiles(P, I, g, B, out, s);
+    launch_forward_tiles(P, I, g, B, out, keep, s);
     return hipGetLastError() == hipSuccess ? PDS_OK : PDS_ERR_HIP;
 }
 
@@ -1247,7 +1285,8 @@ int binned_backward(const float* grad, const PhotonArgs& P, const ImageArgs& I, 
     const Buffers B = buffers(ws, L);
     // no image output: the scatter clears nothing
     if (!reuse) launch_build(P, I, g, B, /*keep=*/true, nullptr, s);
-    launch_backward_tiles(grad, P, I, g, B, G, s);
+    // reuse: the forward of these bins also left every segment's walk order
+    launch_backward_tiles(grad, P, I, g, B, G, reuse, s);
     return hipGetLastError() == hipSuccess ? PDS_OK : PDS_ERR_HIP;
 }
 

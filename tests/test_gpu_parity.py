@@ -334,3 +334,19 @@ def test_chord_walk_equals_box_walk(case, tile):
     for a, b in zip(outs[0][1], outs[1][1]):
         assert torch.equal(a, b), "gradients differ between chord and box walks"
     assert_image_close(outs[1][0], O.forward_f64(*p.arrays(), p.size, p.cap), f"{case} chord walk")
+
+
+@pytest.mark.parametrize("mode", ["auto", "deterministic"])
+def test_backward_with_kept_bins_equals_stateless(mode):
+    # the backward that reuses the forward's bins (and its segment walk order)
+    # must give exactly the gradients of the stateless backward, which bins and
+    # sorts again: each record's sums do not depend on the walk order
+    from shapefromcaustics_amd.PhotonDifferentialSplatting import _backward, _forward
+    p = WL.caustic(60000, (3, 256, 256), 12, seed=12)
+    g = torch.from_numpy(WL.upstream_grad(p.size, seed=13)).cuda()
+    E, xp, M, cp, radius = _dev(p)
+    _, bins = _forward(E, xp, M, cp, radius, p.size, p.cap, mode, None, keep_bins=True)
+    kept = _backward(g, E, xp, M, cp, radius, p.cap, mode, None, bins=bins)
+    fresh = pds_backward(g, E, xp, M, cp, radius, p.cap, mode=mode)
+    for a, b in zip(kept, fresh):
+        assert torch.equal(a, b)
