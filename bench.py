@@ -168,6 +168,67 @@ def bench_landweber(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_scaleout(args):
+    """Memory-bound scale-out on one GPU: 4 views x 2^24 caustic-line photons
+    into 12 x 2048 x 2048 (cap 16).  A step generates the photons (counter-
+    based source), splats them keeping the bins, drops them, regenerates them
+    for the backward (recomputation instead of storage) and runs the adjoint.
+    Timed in the atomic and the deterministic mode."""
+    import torch
+
+    from shapefromcaustics_amd.generate import ScaleOut
+    from shapefromcaustics_amd.PhotonDifferentialSplatting import _backward, _forward
+
+    torch.cuda.set_device(0)
+    so = ScaleOut()
+    grad = torch.randn(so.size, device="cuda", generator=torch.Generator("cuda").manual_seed(11))
+
+    def step(mode):
+        ph = so.photons()
+        img, bins = _forward(*ph, so.size, so.cap, mode, None, keep_bins=True)
+        del ph
+        ph = so.photons()
+        g = _backward(grad, *ph, so.cap, mode, None, bins=bins)
+        return img, g
+
+    res = {}
+    for mode in ("auto", "deterministic"):
+        for _ in range(args.warmup):
+            step(mode)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step(mode)
+        torch.cuda.synchronize()
+        res[mode] = (time.perf_counter() - t0) / args.steps
+        log(f"[bench] scaleout {mode}: {res[mode] * 1e3:.2f} ms/step ({so.n / res[mode]:.3e} photons/s)")
+    plain = None
+    if not args.no_plain:
+        ph = so.photons()
+        for _ in range(1):
+            img = _forward(*ph, so.size, so.cap, "plain", None, keep_bins=False)[0]
+            _backward(grad, *ph, so.cap, "plain", None)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        img = _forward(*ph, so.size, so.cap, "plain", None, keep_bins=False)[0]
+        _backward(grad, *ph, so.cap, "plain", None)
+        torch.cuda.synchronize()
+        plain = time.perf_counter() - t0
+        log(f"[bench] scaleout plain-atomic (photons kept, no generation): {plain * 1e3:.2f} ms/step")
+    wall = res["auto"]
+    line = {
+        "metric": "photons/sec fwd+bwd, scale-out (4 views x 2^24 -> 12x2048x2048, cap 16, photons regenerated)",
+        "value": so.n / wall, "unit": "photons/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (counter-based caustic-line photons, N(0,1) upstream gradient)",
+        "config": {"workload": "scaleout: 4 views x 2^24 photons -> 12x2048x2048 fp32, cap 16, caustic lines; "
+                               "photons generated on the GPU and regenerated for the backward",
+                   "photons": so.n, "deterministic_ms_per_step": res["deterministic"] * 1e3,
+                   "plain_atomic_ms_per_step": None if plain is None else plain * 1e3},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -195,6 +256,8 @@ def main():
 
     if args.workload == "landweber":
         return bench_landweber(args)
+    if args.workload == "scaleout":
+        return bench_scaleout(args)
     if args.workload == "copy":
         # calibration for HBM byte counters: 5 device-to-device copies of 1 GiB
         # (1048576 KiB read and written per copy)

@@ -15,7 +15,7 @@ REPO_DIR = os.path.dirname(PKG_DIR)
 CSRC = os.path.join(PKG_DIR, "csrc")
 LIB_DIR = os.path.join(PKG_DIR, "lib")
 LIB_PATH = os.path.join(LIB_DIR, "libpds_hip.so")
-SOURCES = ["pds_api.hip", "pds_kernels.hip", "pds_binned.hip", "hf_kernels.hip"]
+SOURCES = ["pds_api.hip", "pds_kernels.hip", "pds_binned.hip", "hf_kernels.hip", "gen_kernels.hip"]
 ARCH = "gfx950"
 
 

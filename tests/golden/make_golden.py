@@ -13,7 +13,9 @@ caught against fixed data.  Cases:
                    gradient arrays (gE, gx, gM) with their |term| magnitudes.
 * heightfield.npz  a 12x12 height field, 16x16 parallel rays: Sobel normals,
                    the splat inputs of every photon that reaches the sensor,
-                   the 32x32 image, the loss against a fixed target and dL/dh.
+                   the rays kept away from triangle edges, and over those
+                   rays the 32x32 image, the loss against a fixed target and
+                   dL/dh.
 """
 import os
 import sys
@@ -59,12 +61,16 @@ def heightfield_case():
     h = torch.from_numpy(HO.gaussian_bumps(R, seed=31, n_bumps=5, amp=(0.0, 0.06), sigma=(0.2, 0.5))).float().double()
     rays = HO.parallel_rays(n_side, 1.0)
     target = torch.from_numpy(np.random.default_rng(32).uniform(0, 1, size))
-    L, g, img, path = HO.loss_and_grad(h, rays, 1.5, sz, size, cap, target)
+    with torch.no_grad():
+        safe = HO.edge_safe_rays(HO.simulate(h, rays, 1.5, sz), n_side * n_side)
+    # image, loss and dL/dh of the photons whose hits keep away from triangle
+    # edges (where an FP32 trace may pick the neighbouring, equally valid triangle)
+    L, g, img, path = HO.loss_and_grad(h, rays, 1.5, sz, size, cap, target, keep_rays=safe)
     normals = HO.sobel_normals(h, 2.0 / (R - 1)).permute(1, 2, 0)
     np.savez_compressed(os.path.join(HERE, "heightfield.npz"), h=h.numpy(), normals=normals.numpy(),
                         ray=path.ray.numpy(), Ep=path.Ep.detach().numpy(), xp=path.xp.detach().numpy(),
                         Mp=path.Mp.detach().numpy(), radius=path.radius.detach().numpy(), image=img.numpy(),
-                        target=target.numpy(), loss=np.array(L), grad_h=g.numpy(),
+                        target=target.numpy(), loss=np.array(L), grad_h=g.numpy(), safe=safe.numpy(),
                         meta=np.array([R, n_side, size[1], cap]), sensor_z=np.array(sz), ior=np.array(1.5))
 
 

@@ -59,7 +59,7 @@ def test_heightfield_oracle_reproduces_golden():
     h = torch.from_numpy(z["h"])
     rays = HO.parallel_rays(n_side, 1.0)
     L, g, img, path = HO.loss_and_grad(h, rays, float(z["ior"]), float(z["sensor_z"]), (1, S, S), cap,
-                                       torch.from_numpy(z["target"]))
+                                       torch.from_numpy(z["target"]), keep_rays=torch.from_numpy(z["safe"]))
     assert np.array_equal(path.ray.numpy(), z["ray"])
     assert np.allclose(path.Ep.detach().numpy(), z["Ep"], rtol=1e-12)
     assert np.allclose(img.numpy(), z["image"], rtol=1e-11, atol=1e-14)
@@ -95,16 +95,22 @@ def test_gpu_heightfield_matches_golden():
     h = torch.from_numpy(z["h"]).float().cuda()
     assert np.abs(HF.normals(h).cpu().numpy() - z["normals"]).max() < 1e-6
     sc, em = HF.Scene(R, float(z["sensor_z"]), (float(z["ior"]),)), HF.ParallelEmitter(n_side, 1.0)
-    Ep, xp, Mp, cp, radius = [a.cpu().numpy() for a in HF.simulate(h, sc, em)]
-    alive = np.nonzero(cp[0] >= 0)[0]
+    from shapefromcaustics_amd.PhotonDifferentialSplatting import _backward, _forward
+    n = HF.normals(h)
+    Ep, xp, Mp, cp, radius = HF.simulate(h, sc, em, n)
+    alive = np.nonzero(cp[0].cpu().numpy() >= 0)[0]
     assert np.array_equal(alive, z["ray"])
-    assert np.abs(Ep[alive] - z["Ep"]).max() <= 1e-4 * np.abs(z["Ep"]).max()
-    hh = h.clone().requires_grad_(True)
-    img = HF.CausticImage.apply(hh, sc, em, (1, S, S), cap)
-    loss = 0.5 * ((img - torch.from_numpy(z["target"]).float().cuda()) ** 2).sum()
-    loss.backward()
+    safe = z["safe"]
+    ks = safe[z["ray"]]
+    assert np.abs(Ep.cpu().numpy()[alive][ks] - z["Ep"][ks]).max() <= 1e-4 * np.abs(z["Ep"]).max()
+    # image and dL/dh over the edge-safe photons (others get channel -1)
+    cps = torch.where(torch.from_numpy(safe).cuda()[None], cp, torch.full_like(cp, -1))
+    img, bins = _forward(Ep, xp, Mp, cps, radius, (1, S, S), cap, "auto", None, keep_bins=True)
+    resid = img - torch.from_numpy(z["target"]).float().cuda()
+    gE, gx, gM = _backward(resid, Ep, xp, Mp, cps, radius, cap, "auto", None, bins=bins)
+    gh = HF.adjoint(h, sc, em, gE, gx, gM, n).cpu().numpy()
     ref = z["image"]
     mm = ref > 1e-3 * ref.max()
-    assert (np.abs(img.detach().cpu().numpy() - ref)[mm] / ref[mm]).max() <= 1e-4
+    assert (np.abs(img.cpu().numpy() - ref)[mm] / ref[mm]).max() <= 1e-4
     gr = z["grad_h"]
-    assert np.abs(hh.grad.cpu().numpy() - gr).max() <= 1e-3 * np.abs(gr).max()
+    assert np.abs(gh - gr).max() <= 1e-3 * np.abs(gr).max()
