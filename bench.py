@@ -229,6 +229,52 @@ def bench_scaleout(args):
     print(json.dumps(line), flush=True)
 
 
+def summarize_profile(path, factor_fetch=1.0, factor_write=1.0):
+    """Markdown per-kernel table of a rocprofv3 SQLite result (kernel trace
+    and/or --pmc counters; FETCH_SIZE / WRITE_SIZE scaled by the given copy
+    calibration factors and shown in MB per dispatch)."""
+    import sqlite3
+    from collections import defaultdict
+    db = sqlite3.connect(path)
+    cur = db.cursor()
+    views = {r[0] for r in cur.execute("select name from sqlite_master where type in ('table','view')")}
+    clean = lambda k: k.replace("(anonymous namespace)::", "").split("(")[0]
+    if "kernels" in views:
+        cols = [r[1] for r in cur.execute("pragma table_info(kernels)")]
+        ki = cols.index("name") if "name" in cols else cols.index("kernel_name")
+        si, ei = cols.index("start"), cols.index("end")
+        d = defaultdict(list)
+        for r in cur.execute("select * from kernels"):
+            d[r[ki]].append((r[ei] - r[si]) / 1e3)
+        print("| kernel | calls | avg us | total us |\n|---|---:|---:|---:|")
+        for k, v in sorted(d.items(), key=lambda kv: -sum(kv[1])):
+            print(f"| `{clean(k)}` | {len(v)} | {sum(v) / len(v):.1f} | {sum(v):.1f} |")
+    view = next((v for v in ("counters", "pmc_events") if v in views), None)
+    if view is not None:
+        cols = [r[1] for r in cur.execute(f"pragma table_info({view})")]
+        col = lambda *names: next(cols.index(n) for n in names if n in cols)
+        ki, ci, vi, di = col("kernel_name", "name"), col("counter_name"), col("value", "counter_value"), \
+            col("dispatch_id", "id")
+        agg = defaultdict(lambda: defaultdict(float))
+        nd = defaultdict(set)
+        for r in cur.execute(f"select * from {view}"):
+            agg[r[ki]][r[ci]] += r[vi]
+            nd[r[ki]].add(r[di])
+        names = sorted({c for dd in agg.values() for c in dd})
+        print("\n| kernel | dispatches | " + " | ".join(names) + " |\n|---|---:|" + "---:|" * len(names))
+        for k, dd in agg.items():
+            n = len(nd[k])
+            vals = []
+            for c in names:
+                v = dd.get(c, 0.0) / n
+                if c == "FETCH_SIZE":
+                    v = v * factor_fetch * 1024 / 1e6
+                elif c == "WRITE_SIZE":
+                    v = v * factor_write * 1024 / 1e6
+                vals.append(f"{v:.4g}")
+            print(f"| `{clean(k)}` | {n} | " + " | ".join(vals) + " |")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -244,7 +290,12 @@ def main():
                     help="capture the step in a HIP graph and time its replays (default for --workload batched)")
     ap.add_argument("--eager", action="store_true", help="never use a HIP graph")
     ap.add_argument("--tuned", action="store_true", help="use the autotuner's cached tuning for this problem")
+    ap.add_argument("--summarize", metavar="DB", help="print a Markdown summary of a rocprofv3 result and exit")
+    ap.add_argument("--fetch-factor", type=float, default=1.0, help="FETCH_SIZE calibration factor (--summarize)")
+    ap.add_argument("--write-factor", type=float, default=1.0, help="WRITE_SIZE calibration factor (--summarize)")
     args = ap.parse_args()
+    if args.summarize:
+        return summarize_profile(args.summarize, args.fetch_factor, args.write_factor)
     if args.workload == "batched" and not args.eager:
         args.graph = True
     if args.eager:
