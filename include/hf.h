@@ -51,16 +51,29 @@ typedef struct hf_scene {
     int32_t reserved[4];
 } hf_scene;
 
-/* Parallel emitter: vertical rays (0, 0, -1) from height z0 through the
- * centres of an n_side x n_side grid over [-1, 1]^2, one ray per channel per
- * grid cell (ray i: cell i / channels, row-major), Du = 2 pitch e_x,
- * Dv = 2 pitch e_y, Dtheta = Dphi = 0, flux flux_total / n_side^2 each.
- * N = n_side^2 * channels photons. */
+/* Photon source; photon i is sample i / channels in channel i mod channels,
+ * N = n_side^2 * channels photons.
+ * kind 0, parallel: vertical rays (0, 0, -1) from height z0 through the
+ *   centres of an n_side x n_side grid over [-1, 1]^2 (sample s: row s /
+ *   n_side), Du = 2 pitch e_x, Dv = 2 pitch e_y, Dtheta = Dphi = 0, flux
+ *   flux_total / n_side^2 each.
+ * kind 1, point light (model/caustics.py:26-70): rays from light[] whose
+ *   directions are bilinear slerps, with coefficients (a, b) drawn per sample
+ *   from (seed, sample) by SplitMix64, of the directions to the corners of the
+ *   square [-1, 1]^2 at height plane_z; Du = Dv = 0; Dtheta, Dphi = Frisvad's
+ *   orthonormal basis of the direction (model/utils.py:33-53) times
+ *   diff_scale (the reference's 2 smoothing sqrt(solid angle / (pi total)));
+ *   flux flux_total per photon. */
 typedef struct hf_emitter {
     int32_t n_side;
     float z0;
     float flux_total;
-    int32_t reserved[5];
+    int32_t kind;
+    float light[3];
+    float plane_z;
+    float diff_scale;
+    uint32_t seed;
+    int32_t reserved[2];
 } hf_emitter;
 
 /* Message of the calling thread's last failed hf_* call. */

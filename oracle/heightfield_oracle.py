@@ -283,6 +283,53 @@ def parallel_rays(n_side: int, z0: float, flux_total=1.0, rows=None) -> Rays:
                 torch.zeros(n, dtype=torch.int64))
 
 
+def slerp(p0, p1, t):
+    """model/utils.py:84-88."""
+    w = torch.acos(dot(p0, p1, keepdim=True).clamp(-1, 1))
+    return (torch.sin((1 - t) * w) * p0 + torch.sin(t * w) * p1) / torch.sin(w)
+
+
+def frisvad_basis(n, eps=1e-5):
+    """model/utils.py:33-53 (Frisvad 2012), n [N, 3] unit."""
+    nx, ny, nz = n[:, 0], n[:, 1], n[:, 2]
+    sing = nz < (-1 + eps)
+    a = 1 / (1 + torch.where(sing, torch.zeros_like(nz), nz))
+    b = -nx * ny * a
+    b1 = torch.stack([1 - a * nx ** 2, b, -nx], -1)
+    b2 = torch.stack([b, 1 - a * ny ** 2, -ny], -1)
+    b1 = torch.where(sing[:, None], torch.tensor([0.0, -1.0, 0.0], dtype=F64), b1)
+    b2 = torch.where(sing[:, None], torch.tensor([-1.0, 0.0, 0.0], dtype=F64), b2)
+    return b1, b2
+
+
+def point_light_rays(n_side, channels, light, plane_z, diff_scale, flux, seed) -> Rays:
+    """model/caustics.py:26-70 with the interpolation coefficients (a, b) of
+    sample s drawn as SplitMix64(seed * 0x100000001B3 ^ (s << 2 | k)), k = 0, 1
+    (the GPU emitter's counter-based draws); photon i is sample i // channels
+    in channel i % channels."""
+    from oracle import gen_twin as GT
+    n_s = n_side * n_side
+    with np.errstate(over="ignore"):
+        key = (np.uint64(seed) * np.uint64(0x100000001B3)) & GT.M64
+    ctr = np.arange(n_s, dtype=np.uint64) << np.uint64(2)
+    a = torch.from_numpy(GT.u24(GT.splitmix64(key ^ (ctr | np.uint64(0)))).astype(np.float64))[:, None]
+    b = torch.from_numpy(GT.u24(GT.splitmix64(key ^ (ctr | np.uint64(1)))).astype(np.float64))[:, None]
+    L = torch.tensor(light, dtype=F64)
+
+    def corner(x, y):
+        return normalize(torch.tensor([x, y, plane_z], dtype=F64) - L)[None].expand(n_s, 3)
+
+    lower = slerp(corner(-1, -1), corner(1, -1), a)
+    upper = slerp(corner(-1, 1), corner(1, 1), a)
+    d = slerp(lower, upper, b)
+    b1, b2 = frisvad_basis(d)
+    rep = lambda t: t.repeat_interleave(channels, 0)
+    n = n_s * channels
+    z = torch.zeros(n, 3, dtype=F64)
+    return Rays(L[None].expand(n, 3).clone(), rep(d), z, z.clone(), rep(diff_scale * b1), rep(diff_scale * b2),
+                torch.full((n,), float(flux), dtype=F64), torch.arange(n) % channels)
+
+
 # ----------------------------------------------------------------------------- the bounce loop
 
 @dataclasses.dataclass

@@ -130,9 +130,17 @@ __global__ __launch_bounds__(256) void trace_kernel(SceneArgs S, const float* __
 // ---------------------------------------------------------------------------
 // the photon path
 // ---------------------------------------------------------------------------
+struct D3 {
+    double x, y, z;
+};
+
 struct EmitArgs {
-    int32_t n_side, channels;
+    int32_t n_side, channels, kind;
     float z0, pitch, flux;
+    V3 light;
+    D3 c00, c10, c01, c11;  // point light: unit directions to the plane's corners
+    double diff_scale;
+    uint64_t key;
     float ior[HF_MAX_CHANNELS];
 };
 
@@ -140,11 +148,65 @@ struct PathState {
     V3 o, d, Du, Dv, Dt, Dp;
 };
 
+__device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
+    // SplitMix64 (Steele, Lea and Flood, OOPSLA 2014)
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ float u24(uint64_t bits) { return (float)(uint32_t)(bits >> 40) * (1.0f / 16777216.0f); }
+
+// The point-light emitter works in double: Frisvad's basis loses
+// 1 / (1 + n_z) of relative accuracy for directions near -z (rays from a light
+// above the glass), so the direction must be exact well below float rounding
+// before the basis is formed; results are rounded to float once.
+
+// model/utils.py:84-88: (sin((1 - t) w) p0 + sin(t w) p1) / sin(w), w = acos(clamp(p0 . p1))
+__device__ __forceinline__ D3 slerp_d(D3 p0, D3 p1, double t) {
+    const double c = p0.x * p1.x + p0.y * p1.y + p0.z * p1.z;
+    const double w = acos(fmin(fmax(c, -1.0), 1.0));
+    const double sw = sin(w), a = sin((1.0 - t) * w) / sw, b = sin(t * w) / sw;
+    return D3{a * p0.x + b * p1.x, a * p0.y + b * p1.y, a * p0.z + b * p1.z};
+}
+
+// Frisvad 2012 orthonormal basis of a unit vector (model/utils.py:33-53)
+__device__ __forceinline__ void frisvad_d(D3 n, D3& b1, D3& b2) {
+    if (n.z < -1.0 + 1e-5) {
+        b1 = D3{0.0, -1.0, 0.0};
+        b2 = D3{-1.0, 0.0, 0.0};
+        return;
+    }
+    const double a = 1.0 / (1.0 + n.z), b = -n.x * n.y * a;
+    b1 = D3{1.0 - a * n.x * n.x, b, -n.x};
+    b2 = D3{b, 1.0 - a * n.y * n.y, -n.y};
+}
+
+__device__ __forceinline__ V3 to_v3(D3 v, double s) { return v3((float)(s * v.x), (float)(s * v.y), (float)(s * v.z)); }
+
 __device__ __forceinline__ PathState emit(const EmitArgs& E, int64_t i, int& ch) {
     ch = (int)(i % E.channels);
     const int64_t cell = i / E.channels;
-    const int row = (int)(cell / E.n_side), col = (int)(cell - (int64_t)row * E.n_side);
     PathState s;
+    if (E.kind == 1) {
+        // model/caustics.py:26-70: slerp between the corner directions with
+        // per-sample coefficients (a, b), Du = Dv = 0, scaled orthonormal Dtheta, Dphi
+        const uint64_t ctr = (uint64_t)cell << 2;
+        const double a = u24(splitmix64(E.key ^ (ctr | 0))), b = u24(splitmix64(E.key ^ (ctr | 1)));
+        const D3 lower = slerp_d(E.c00, E.c10, a), upper = slerp_d(E.c01, E.c11, a);
+        const D3 d = slerp_d(lower, upper, b);
+        D3 b1, b2;
+        frisvad_d(d, b1, b2);
+        s.d = to_v3(d, 1.0);
+        s.o = E.light;
+        s.Dt = to_v3(b1, E.diff_scale);
+        s.Dp = to_v3(b2, E.diff_scale);
+        s.Du = v3(0.f, 0.f, 0.f);
+        s.Dv = v3(0.f, 0.f, 0.f);
+        return s;
+    }
+    const int row = (int)(cell / E.n_side), col = (int)(cell - (int64_t)row * E.n_side);
     s.o = v3(-1.0f + (float)(2 * col + 1) / (float)E.n_side, -1.0f + (float)(2 * row + 1) / (float)E.n_side, E.z0);
     s.d = v3(0.f, 0.f, -1.f);
     s.Du = v3(2.0f * E.pitch, 0.f, 0.f);
@@ -694,9 +756,24 @@ EmitArgs emit_args(const hf_scene* sc, const hf_emitter* em) {
     EmitArgs E;
     E.n_side = em->n_side;
     E.channels = sc->channels;
+    E.kind = em->kind;
     E.z0 = em->z0;
     E.pitch = (float)(2.0 / (double)em->n_side);
-    E.flux = (float)((double)em->flux_total / ((double)em->n_side * (double)em->n_side));
+    E.flux = em->kind == 1 ? em->flux_total
+                           : (float)((double)em->flux_total / ((double)em->n_side * (double)em->n_side));
+    const double L[3] = {em->light[0], em->light[1], em->light[2]};
+    auto corner = [&](double x, double y) {
+        const double dx = x - L[0], dy = y - L[1], dz = (double)em->plane_z - L[2];
+        const double n = std::sqrt(dx * dx + dy * dy + dz * dz);
+        return D3{dx / n, dy / n, dz / n};
+    };
+    E.light = V3{em->light[0], em->light[1], em->light[2]};
+    E.c00 = corner(-1, -1);
+    E.c10 = corner(1, -1);
+    E.c01 = corner(-1, 1);
+    E.c11 = corner(1, 1);
+    E.diff_scale = em->diff_scale;
+    E.key = (uint64_t)em->seed * 0x100000001B3ull;
     for (int k = 0; k < HF_MAX_CHANNELS; ++k) E.ior[k] = sc->ior[k];
     return E;
 }
@@ -712,6 +789,7 @@ int check_scene(const hf_scene* sc, const void* h, const void* normals) {
 int check_emitter(const hf_emitter* em) {
     if (!em) return fail(PDS_ERR_ARGUMENT, "null emitter");
     if (em->n_side < 1 || em->n_side > 65536) return fail(PDS_ERR_ARGUMENT, "emitter n_side must be in [1, 65536]");
+    if (em->kind != 0 && em->kind != 1) return fail(PDS_ERR_ARGUMENT, "emitter kind must be 0 (parallel) or 1 (point)");
     return PDS_OK;
 }
 

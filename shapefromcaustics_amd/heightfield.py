@@ -19,6 +19,7 @@ from __future__ import annotations
 
 import ctypes
 import dataclasses
+import math
 import typing as tp
 
 import torch
@@ -36,7 +37,8 @@ class HfScene(ctypes.Structure):
 
 
 class HfEmitter(ctypes.Structure):
-    _fields_ = [("n_side", _i32), ("z0", _f32), ("flux_total", _f32), ("reserved", _i32 * 5)]
+    _fields_ = [("n_side", _i32), ("z0", _f32), ("flux_total", _f32), ("kind", _i32), ("light", _f32 * 3),
+                ("plane_z", _f32), ("diff_scale", _f32), ("seed", ctypes.c_uint32), ("reserved", _i32 * 2)]
 
 
 class HfLandweber(ctypes.Structure):
@@ -121,6 +123,55 @@ class ParallelEmitter:
         return e
 
 
+def subtended_angle(light, tris) -> float:
+    """Solid angle of triangles [T, 3, 3] seen from ``light`` (Oosterom and
+    Strackee, as model/utils.py:91-108: 2 atan2(d0 . (d2 x d1), |d0||d1||d2| +
+    |d0| d1.d2 + |d1| d0.d2 + |d2| d0.d1), negative (back-facing) ones dropped)."""
+    import numpy as np
+    total = 0.0
+    for t in np.asarray(tris, np.float64):
+        d0, d1, d2 = (t[k] - np.asarray(light, np.float64) for k in range(3))
+        n0, n1, n2 = (np.linalg.norm(v) for v in (d0, d1, d2))
+        num = float(np.dot(d0, np.cross(d2, d1)))
+        den = n0 * n1 * n2 + n0 * float(np.dot(d1, d2)) + n1 * float(np.dot(d0, d2)) + n2 * float(np.dot(d0, d1))
+        if not (num == 0 and den == 0):
+            total += max(2.0 * math.atan2(num, den), 0.0)
+    return total
+
+
+@dataclasses.dataclass
+class PointLight:
+    """Point light at ``light`` aiming through the square [-1, 1]^2 at height
+    ``plane_z`` (model/caustics.py:26-70): n_side^2 samples, flux per photon,
+    Dtheta/Dphi scaled by 2 smoothing sqrt(solid angle / (pi total photons))."""
+
+    n_side: int
+    light: tp.Sequence[float]
+    plane_z: float
+    flux: float = 1e-5
+    smoothing: float = 1.0
+    seed: int = 0
+    total_photons: tp.Optional[int] = None  # defaults to this emitter's photon count
+
+    def diff_scale(self, channels: int = 1) -> float:
+        z = self.plane_z
+        tris = [[[-1, -1, z], [1, -1, z], [-1, 1, z]], [[-1, 1, z], [1, -1, z], [1, 1, z]]]
+        total = self.total_photons or self.n_side ** 2 * channels
+        return 2.0 * self.smoothing * math.sqrt(subtended_angle(self.light, tris) / (math.pi * total))
+
+    def struct(self, channels: int = 1) -> HfEmitter:
+        e = HfEmitter()
+        e.n_side, e.kind, e.flux_total = int(self.n_side), 1, float(self.flux)
+        for k in range(3):
+            e.light[k] = float(self.light[k])
+        e.plane_z, e.diff_scale, e.seed = float(self.plane_z), float(self.diff_scale(channels)), int(self.seed)
+        return e
+
+
+def _emitter_struct(emitter, scene) -> HfEmitter:
+    return emitter.struct(len(scene.iors)) if isinstance(emitter, PointLight) else emitter.struct()
+
+
 def _check_h(h: torch.Tensor, R: int):
     if not h.is_cuda or h.dtype != torch.float32 or not h.is_contiguous() or tuple(h.shape) != (R, R):
         raise RuntimeError(f"height field must be a contiguous float32 CUDA tensor of shape ({R}, {R})")
@@ -166,7 +217,7 @@ def simulate(h: torch.Tensor, scene: Scene, emitter: ParallelEmitter, n: tp.Opti
     Ep, xp, Mp = torch.empty(N, **f32), torch.empty(2, N, **f32), torch.empty(2, 3, N, **f32)
     cp, radius = torch.empty(1, N, dtype=torch.int64, device=dev), torch.empty(N, **f32)
     ws = _ws(scene.R, dev)
-    sc, em = scene.struct(), emitter.struct()
+    sc, em = scene.struct(), _emitter_struct(emitter, scene)
     _check(lib().hf_simulate_f32(_p(h), _p(n), ctypes.byref(sc), ctypes.byref(em), _p(Ep), _p(xp), _p(Mp), _p(cp),
                                  _p(radius), _p(ws), ws.numel(), _stream(dev)), "hf_simulate_f32")
     return Ep, xp, Mp, cp, radius
@@ -180,7 +231,7 @@ def adjoint(h: torch.Tensor, scene: Scene, emitter: ParallelEmitter, gE, gx, gM,
     n = normals(h) if n is None else n
     out = torch.empty_like(h)
     ws = _ws(scene.R, dev)
-    sc, em = scene.struct(), emitter.struct()
+    sc, em = scene.struct(), _emitter_struct(emitter, scene)
     _check(lib().hf_adjoint_f32(_p(h), _p(n), ctypes.byref(sc), ctypes.byref(em), _p(gE.contiguous()),
                                 _p(gx.contiguous()), _p(gM.contiguous()), _p(out), _p(ws), ws.numel(),
                                 _stream(dev)), "hf_adjoint_f32")

@@ -13,6 +13,7 @@ import pytest
 import torch
 
 from oracle import heightfield_oracle as HO
+from oracle import pds_oracle as O
 from shapefromcaustics_amd import heightfield as HF
 
 pytestmark = pytest.mark.gpu
@@ -188,3 +189,53 @@ def test_landweber_step_runs_end_to_end(problem):
     want = HO.landweber_update(d0.cpu().double(), gh.cpu().double(), mask.cpu().double(), 0.1, 2e-3, 0.0, 0.3,
                                -math.inf, math.inf, 0.1, 2)
     assert (d.cpu().double() - want).abs().max() < 1e-5
+
+
+@pytest.mark.parametrize("light", [(0.1, -0.2, 3.0), (-0.5, 0.4, 2.0)])
+def test_point_light_emitter_and_adjoint(light):
+    # model/caustics.py:26-70: rays from a point light with direction
+    # differentials (Du = Dv = 0); their transfer, refraction and adjoint
+    R, n_side, sz, size, cap = 16, 40, -1.5, (1, 64, 64), 8
+    h = _h(R, 9, n_bumps=6, amp=(0.0, 0.06), sigma=(0.2, 0.5)).float()
+    em = HF.PointLight(n_side, light, 0.2, flux=1e-3, smoothing=1.0, seed=7)
+    rays = HO.point_light_rays(n_side, 1, light, 0.2, em.diff_scale(1), em.flux, em.seed)
+    with torch.no_grad():
+        base = HO.simulate(h.double(), rays, 1.5, sz)
+    safe = HO.edge_safe_rays(base, n_side * n_side)
+    target = torch.from_numpy(np.random.default_rng(3).uniform(0, 1e-3, size))
+    L, g, img, path = HO.loss_and_grad(h.double(), rays, 1.5, sz, size, cap, target, keep_rays=safe)
+    assert path.ray.numel() > 1000 and (path.radius > 0).all()
+    P = dict(R=R, n_side=n_side, sz=sz, h=h, size=size, cap=cap, target=target, safe=safe)
+    from shapefromcaustics_amd.PhotonDifferentialSplatting import _backward, _forward
+    sc = HF.Scene(R, sz)
+    hc = h.cuda()
+    n = HF.normals(hc)
+    Ep, xp, Mp, cp, radius = HF.simulate(hc, sc, em, n)
+    alive = torch.zeros(n_side * n_side, dtype=torch.bool)
+    alive[base.ray] = True
+    assert torch.equal((cp[0] >= 0).cpu(), alive)
+    keep = safe[path.ray]
+    r = path.ray[keep]
+    rel = lambda a, b: float((a.cpu().double() - b.detach()).abs().max() / b.detach().abs().max())
+    assert rel(Ep[r], path.Ep[keep]) <= 1e-4 and rel(Mp[:, :, r], path.Mp[:, :, keep]) <= 1e-4
+    assert rel(radius[r], path.radius[keep]) <= 1e-4
+    cps = torch.where(safe.cuda()[None], cp, torch.full_like(cp, -1))
+    im, bins = _forward(Ep, xp, Mp, cps, radius, size, cap, "auto", None, keep_bins=True)
+    resid = im - target.float().cuda()
+    gE, gx, gM = _backward(resid, Ep, xp, Mp, cps, radius, cap, "auto", None, bins=bins)
+    gh = HF.adjoint(hc, sc, em, gE, gx, gM, n).cpu().double()
+    got = im.cpu().double().numpy()
+    # the splat of these very inputs: the FP64 oracle on the GPU's FP32 splat inputs
+    c_ref = O.forward_f64(*[t.cpu().numpy() for t in (Ep, xp, Mp, cps, radius)], size, cap)
+    mc = c_ref > 1e-3 * c_ref.max()
+    assert (np.abs(got - c_ref)[mc] / c_ref[mc]).max() <= 1e-4
+    # the whole FP32 chain against the FP64 one: the faint caustic edge (pixels
+    # near 1e-3 max, footprint rims where K = (1 - q)^2 is small) moves by
+    # ~1.5e-4 under FP32 rounding of the photon positions and differentials
+    ref = img.numpy()
+    m = ref > 1e-3 * ref.max()
+    assert (np.abs(got - ref)[m] / ref[m]).max() <= 5e-4
+    assert np.linalg.norm(got - ref) <= 1e-5 * np.linalg.norm(ref)
+    err = float((gh - g).abs().max() / g.abs().max())
+    print(f"point light {light}: dL/dh max err / max {err:.3e}")
+    assert err <= 1e-3
