@@ -56,7 +56,7 @@ def cpu_baseline(p, grad, budget_s=12.0):
     threads = O.cpu_threads()
     times = []
     t_start = time.perf_counter()
-    while len(times) < 30 and (time.perf_counter() - t_start) < budget_s:
+    while len(times) < 60 and (time.perf_counter() - t_start) < budget_s:
         t0 = time.perf_counter()
         O.cpu_forward_f32(*p.arrays(), p.size, p.cap)
         O.cpu_backward_f32(grad, *p.arrays(), p.cap)
@@ -261,6 +261,8 @@ def summarize_profile(path, factor_fetch=1.0, factor_write=1.0):
             agg[r[ki]][r[ci]] += r[vi]
             nd[r[ki]].add(r[di])
         names = sorted({c for dd in agg.values() for c in dd})
+        if not names:
+            return
         print("\n| kernel | dispatches | " + " | ".join(names) + " |\n|---|---:|" + "---:|" * len(names))
         for k, dd in agg.items():
             n = len(nd[k])
