@@ -14,19 +14,46 @@ from shapefromcaustics_amd import PhotonDifferentialSplatting as PDSmod
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def _declared_functions():
-    src = open(os.path.join(ROOT, "include", "pds.h")).read()
+def _declared_functions(header, prefix):
+    src = open(os.path.join(ROOT, "include", header)).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b(pds_[a-z0-9_]+)\s*\(", src)))
+    return sorted(set(re.findall(r"\b(" + prefix + r"_[a-z0-9_]+)\s*\(", src)))
 
 
 def test_library_exports_every_declared_symbol():
+    from shapefromcaustics_amd import generate, heightfield
     lib = _native.lib()
-    names = _declared_functions()
+    names = _declared_functions("pds.h", "pds")
     assert {"pds_forward_f32", "pds_backward_f32", "pds_forward_workspace_bytes", "pds_last_error"} <= set(names)
     for name in names:
         assert hasattr(lib, name), name
     assert set(names) <= set(_native.SIGNATURES)
+    hf = _declared_functions("hf.h", "hf")
+    assert {"hf_trace_f32", "hf_simulate_f32", "hf_adjoint_f32", "hf_landweber_f32"} <= set(hf)
+    for name in hf:
+        assert hasattr(lib, name), name
+    assert set(hf) <= set(heightfield.SIGNATURES)
+    gen = _declared_functions("pds_gen.h", "pds_gen")
+    assert gen and set(gen) <= set(generate.SIGNATURES)
+    for name in gen:
+        assert hasattr(lib, name), name
+
+
+def test_heightfield_and_generator_argument_errors():
+    from shapefromcaustics_amd import generate, heightfield
+    hl, gl = heightfield.lib(), generate.lib()
+    sc = heightfield.Scene(1, -1.0).struct()                       # R < 2
+    rc = hl.hf_simulate_f32(None, None, ctypes.byref(sc), None, None, None, None, None, None, None, 0, None)
+    assert rc == _native.PDS_ERR_ARGUMENT and b"R must be" in hl.hf_last_error()
+    sc = heightfield.Scene(8, -1.0).struct()
+    em = heightfield.ParallelEmitter(4, 1.0).struct()
+    em.kind = 7
+    h = ctypes.c_void_p(1)
+    rc = hl.hf_adjoint_f32(h, h, ctypes.byref(sc), ctypes.byref(em), None, None, None, None, None, 0, None)
+    assert rc == _native.PDS_ERR_ARGUMENT and b"kind" in hl.hf_last_error()
+    rc = gl.pds_gen_caustic_f32(1, 0, 10, 64, 64, None, 4, None, None, None, None, None, 10, 5, None)
+    assert rc == _native.PDS_ERR_ARGUMENT and b"invalid" in gl.pds_gen_last_error()
+    assert hl.hf_workspace_bytes(1) == 0 and hl.hf_workspace_bytes(16) > 16 * 16 * 4
 
 
 def test_version_and_empty_error():
