@@ -111,7 +111,7 @@ bool make_geom(int64_t N, int32_t C, int32_t H, int32_t W, int32_t cap, int32_t 
     g.seg = (t && t->seg_photons > 0) ? t->seg_photons : kDefaultSeg;
     if (g.seg > kMaxSeg) g.seg = kMaxSeg;
     const int64_t bp = (t && t->block_photons > 0) ? t->block_photons : 0;
-    int64_t G = bp > 0 ? (N + bp - 1) / bp : (N + 8191) / 8192;
+    int64_t G = bp > 0 ? (N + bp - 1) / bp : (N + 4095) / 4096;
     if (G > kMaxBinBlocks) G = kMaxBinBlocks;
     if (G < 1) G = 1;
     g.G = (int32_t)G;
@@ -290,11 +290,14 @@ __global__ __launch_bounds__(kBinThreads) void bin_count_kernel(PhotonArgs P, Im
     const int64_t hi = min(lo + g.per_block, P.N);
     float emax = 0.0f;
     int64_t i = lo + threadIdx.x;
-    RawPhoton raw;
-    if (i < hi) raw = load_photon(P, i);
+    // two photons of this lane in flight while one is binned
+    RawPhoton r0 = {}, r1 = {};
+    if (i < hi) r0 = load_photon(P, i);
+    if (i + kBinThreads < hi) r1 = load_photon(P, i + kBinThreads);
     for (; i < hi; i += kBinThreads) {
-        const RawPhoton now = raw;
-        if (i + kBinThreads < hi) raw = load_photon(P, i + kBinThreads);  // prefetch
+        const RawPhoton now = r0;
+        r0 = r1;
+        if (i + 2 * kBinThreads < hi) r1 = load_photon(P, i + 2 * kBinThreads);
         const BinPhoton b = bin_photon(now, I);
         if (!b.rec) continue;
         if (b.e > 0.0f && isfinite(b.e)) emax = fmaxf(emax, b.e);
@@ -446,11 +449,14 @@ __global__ __launch_bounds__(kBinThreads) void bin_scatter_kernel(PhotonArgs P, 
     const int64_t hi = min(lo + g.per_block, P.N);
     const int half = g.T >> 1;
     int64_t i = lo + threadIdx.x;
-    RawPhoton raw;
-    if (i < hi) raw = load_photon(P, i);
+    // two photons of this lane in flight while one is binned
+    RawPhoton r0 = {}, r1 = {};
+    if (i < hi) r0 = load_photon(P, i);
+    if (i + kBinThreads < hi) r1 = load_photon(P, i + kBinThreads);
     for (; i < hi; i += kBinThreads) {
-        const RawPhoton now = raw;
-        if (i + kBinThreads < hi) raw = load_photon(P, i + kBinThreads);  // prefetch
+        const RawPhoton now = r0;
+        r0 = r1;
+        if (i + 2 * kBinThreads < hi) r1 = load_photon(P, i + 2 * kBinThreads);
         const BinPhoton b = bin_photon(now, I);
         int n = 0;
         if (b.rec) {
@@ -610,6 +616,31 @@ struct ChordWalk {
     }
 };
 
+// The walk over a record's whole box rows (the default): the same pixels,
+// order, dx, dy and LDS index as ChordWalk over a full box, with a
+// branch-free row advance (a divergent seek would cost every lane of the
+// wavefront whenever one lane changes row, which is most steps).
+struct BoxWalk {
+    int kx, ky, li, kx0, xb, skip;
+    float dy;
+    __device__ __forceinline__ void start(const TileRec& t, int tshift) {
+        kx0 = t.kx0;
+        kx = t.kx0;
+        ky = t.ky0;
+        xb = t.kx0 + t.bw - 1;
+        li = (t.ly0 << tshift) + t.lx0;
+        skip = (1 << tshift) - t.bw + 1;
+        dy = (float)ky - t.f.fy;
+    }
+    __device__ __forceinline__ void step(const TileRec& t, int, bool) {
+        const bool nl = kx >= xb;
+        kx = nl ? kx0 : kx + 1;
+        li += nl ? skip : 1;
+        ky += nl ? 1 : 0;
+        dy = nl ? (float)ky - t.f.fy : dy;
+    }
+};
+
 __device__ __forceinline__ TileRec decode_tile_rec(const uint4* __restrict__ recs, size_t j, int X0, int Y0, int X1,
                                                    int Y1, int half, bool box_walk) {
     TileRec t;
@@ -639,13 +670,17 @@ __device__ __forceinline__ TileRec decode_tile_rec(const uint4* __restrict__ rec
     t.ly0 = y0 - Y0;
     t.kx0 = x0 - ix;
     t.ky0 = y0 - iy;
-    const Frame& f = t.f;
-    t.A = fmaf(f.a00, f.a00, f.a10 * f.a10);
-    t.P = fmaf(f.a00, f.a01, f.a10 * f.a11);
-    t.det = det2(f.a00, f.a01, f.a10, f.a11);
-    t.invA = 1.0f / t.A;
-    t.full = !(t.A > 1e-30f) || !(t.A < 1e30f) || !(fabsf(t.det) < INFINITY) || !(fabsf(t.P) < INFINITY) ||
-             !(t.invA < INFINITY) || box_walk;
+    t.full = true;  // the box walk visits whole box rows: no chord parameters
+    t.A = t.invA = t.det = t.P = 0.0f;
+    if (!box_walk) {
+        const Frame& f = t.f;
+        t.A = fmaf(f.a00, f.a00, f.a10 * f.a10);
+        t.P = fmaf(f.a00, f.a01, f.a10 * f.a11);
+        t.det = det2(f.a00, f.a01, f.a10, f.a11);
+        t.invA = 1.0f / t.A;
+        t.full = !(t.A > 1e-30f) || !(t.A < 1e30f) || !(fabsf(t.det) < INFINITY) || !(fabsf(t.P) < INFINITY) ||
+                 !(t.invA < INFINITY);
+    }
     return t;
 }
 
@@ -756,7 +791,11 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_kernel(Geom g, PhotonAr
                                                                 float* __restrict__ out,
                                                                 unsigned long long* __restrict__ acc64,
                                                                 uint32_t* __restrict__ order_out) {
-    using Acc = typename std::conditional<DET, unsigned long long, float>::type;
+    // LDS accumulators: int64 fixed point (deterministic) or double.  Both
+    // have fast LDS atomics on gfx950 (ds_add_u64, ds_add_f64), whereas the
+    // single-precision ds_add_f32 serialises its lanes (about 3 cycles per
+    // active lane per CU), which made it the tile kernel's bottleneck.
+    using Acc = typename std::conditional<DET, unsigned long long, double>::type;
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ SegSort S;
     Acc* tile = reinterpret_cast<Acc*>(smem);
@@ -783,24 +822,33 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_kernel(Geom g, PhotonAr
         const int j = S.order[pos];
         const TileRec t = decode_tile_rec(recs, (size_t)d.y + j, st.X0, st.Y0, st.X1, st.Y1, half, g.box_walk);
         const int n = S.key[j] < 255 ? (int)S.key[j] : chord_pixels(t);
-        ChordWalk w;
-        if (n > 0) w.start(t, g.tshift);
-        for (int p = 0; p < n; ++p) {
-            const float dx = (float)w.kx - t.f.fx;
-            float u, v;
-            const float val = silverman(eval_q(t.f, dx, w.dy, u, v)) * t.e;
-            if (val > 0.0f) {
-                if constexpr (DET) {
-                    const unsigned long long fxv = to_fixed(val, fe);
-                    if (fxv >> 62)
-                        atomicOr(&tile[w.li], fxv);
-                    else
-                        atomicAdd(&tile[w.li], fxv);
-                } else {
-                    atomicAdd(&tile[w.li], val);
+        auto walk = [&](auto& w) {
+            for (int p = 0; p < n; ++p) {
+                const float dx = (float)w.kx - t.f.fx;
+                float u, v;
+                const float val = silverman(eval_q(t.f, dx, w.dy, u, v)) * t.e;
+                if (val > 0.0f) {
+                    if constexpr (DET) {
+                        const unsigned long long fxv = to_fixed(val, fe);
+                        if (fxv >> 62)
+                            atomicOr(&tile[w.li], fxv);
+                        else
+                            atomicAdd(&tile[w.li], fxv);
+                    } else {
+                        atomicAdd(&tile[w.li], (double)val);
+                    }
                 }
+                w.step(t, g.tshift, p + 1 < n);
             }
-            w.step(t, g.tshift, p + 1 < n);
+        };
+        if (g.box_walk) {
+            BoxWalk w;
+            w.start(t, g.tshift);
+            walk(w);
+        } else {
+            ChordWalk w;
+            if (n > 0) w.start(t, g.tshift);
+            walk(w);
         }
     }
     __syncthreads();
@@ -817,9 +865,9 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_kernel(Geom g, PhotonAr
                 atomicAdd(&acc64[o], v);  // integer addition: order-independent
         } else {
             if (!split)
-                out[o] = v;
-            else if (v != 0.0f)
-                unsafeAtomicAdd(&out[o], v);
+                out[o] = (float)v;
+            else if (v != 0.0)
+                unsafeAtomicAdd(&out[o], (float)v);
         }
     }
 }
@@ -913,22 +961,31 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_bwd_kernel(Geom g, Imag
         // c = -(12/pi) E, and (u, v) = A (dx, dy), the reference's sums
         // (photon_differentials.cu:195-204) are linear in these moments.
         float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f, m4 = 0.f, m5 = 0.f;
-        ChordWalk cw;
-        if (n > 0) cw.start(t, g.tshift);
-        for (int p = 0; p < n; ++p) {
-            const float dx = (float)cw.kx - t.f.fx, dy = cw.dy;
-            float u, v;
-            const float q = eval_q(t.f, dx, dy, u, v);
-            const float w = q < 1.0f ? 1.0f - q : 0.0f;
-            const float wg = w * gt[cw.li];
-            m0 = fmaf(w, wg, m0);
-            const float px = wg * dx, py = wg * dy;
-            m1 += px;
-            m2 += py;
-            m3 = fmaf(px, dx, m3);
-            m4 = fmaf(px, dy, m4);
-            m5 = fmaf(py, dy, m5);
-            cw.step(t, g.tshift, p + 1 < n);
+        auto walk = [&](auto& cw) {
+            for (int p = 0; p < n; ++p) {
+                const float dx = (float)cw.kx - t.f.fx, dy = cw.dy;
+                float u, v;
+                const float q = eval_q(t.f, dx, dy, u, v);
+                const float w = q < 1.0f ? 1.0f - q : 0.0f;
+                const float wg = w * gt[cw.li];
+                m0 = fmaf(w, wg, m0);
+                const float px = wg * dx, py = wg * dy;
+                m1 += px;
+                m2 += py;
+                m3 = fmaf(px, dx, m3);
+                m4 = fmaf(px, dy, m4);
+                m5 = fmaf(py, dy, m5);
+                cw.step(t, g.tshift, p + 1 < n);
+            }
+        };
+        if (g.box_walk) {
+            BoxWalk cw;
+            cw.start(t, g.tshift);
+            walk(cw);
+        } else {
+            ChordWalk cw;
+            if (n > 0) cw.start(t, g.tshift);
+            walk(cw);
         }
         const Frame& f = t.f;
         const float c = -kTwelveOverPi * t.e;
@@ -1166,7 +1223,7 @@ void launch_forward_tiles(const PhotonArgs& P, const ImageArgs& I, const Geom& g
                           bool keep, hipStream_t s) {
     const unsigned nsegs = (unsigned)g.max_segments;
     // the tile accumulator, or (photon-order path) four waves' walk buffers
-    const size_t tile_lds = std::max((size_t)g.T * g.T * (g.det ? 8 : 4), sizeof(WaveRecs) * (kTileThreads / 64));
+    const size_t tile_lds = std::max((size_t)g.T * g.T * 8, sizeof(WaveRecs) * (kTileThreads / 64));
     if (g.det) {
         hipLaunchKernelGGL(bin_tile_kernel<true>, dim3(nsegs), dim3(kTileThreads), tile_lds, s, g, P, I,
                            (const int4*)B.segs, (const uint4*)B.recs, (const Ctrl*)B.ctrl, out, B.acc64,
