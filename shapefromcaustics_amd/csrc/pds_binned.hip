@@ -641,15 +641,18 @@ struct BoxWalk {
     }
 };
 
-__device__ __forceinline__ TileRec decode_tile_rec(const uint4* __restrict__ recs, size_t j, int X0, int Y0, int X1,
-                                                   int Y1, int half, bool box_walk) {
-    TileRec t;
+// The record's frame and energy for the tile with origin (X0, Y0); also the
+// reference window (radii rx, ry about pixel (pxc, pyc)).
+__device__ __forceinline__ void decode_frame(const uint4* __restrict__ recs, size_t j, int X0, int Y0, int half,
+                                             TileRec& t, int& rx, int& ry, int& pxc, int& pyc) {
     const uint4 q0 = recs[2 * j], q1 = recs[2 * j + 1];
     const uint32_t pk = q0.w;
     const int ix = X0 + half + (int)(int8_t)(pk & 0xff);
     const int iy = Y0 + half + (int)(int8_t)((pk >> 8) & 0xff);
-    const int rx = (pk >> 16) & 63, ry = (pk >> 22) & 63;
-    const int pxc = ix + (int)((pk >> 28) & 3) - 1, pyc = iy + (int)(pk >> 30) - 1;
+    rx = (pk >> 16) & 63;
+    ry = (pk >> 22) & 63;
+    pxc = ix + (int)((pk >> 28) & 3) - 1;
+    pyc = iy + (int)(pk >> 30) - 1;
     t.f.ix = ix;
     t.f.iy = iy;
     t.f.fx = __uint_as_float(q0.x);
@@ -659,17 +662,22 @@ __device__ __forceinline__ TileRec decode_tile_rec(const uint4* __restrict__ rec
     t.f.a01 = __uint_as_float(q1.y);
     t.f.a10 = __uint_as_float(q1.z);
     t.f.a11 = __uint_as_float(q1.w);
-    int x0 = max(max(pxc - rx, X0), 0), x1 = min(pxc + rx, X1);
-    int y0 = max(max(pyc - ry, Y0), 0), y1 = min(pyc + ry, Y1);
-    clip_to_ellipse(t.f, x0, x1, y0, y1);
-    const bool any = x0 <= x1 && y0 <= y1;
-    t.bw = any ? x1 - x0 + 1 : 1;
-    t.rows = any ? y1 - y0 + 1 : 0;
-    t.area = t.bw * t.rows;
+}
+
+// Box fields from the box's first pixel (x0, y0), width and height (rows = 0:
+// empty box, which no walk visits).
+__device__ __forceinline__ void set_tile_box(TileRec& t, int x0, int y0, int bw, int rows, int X0, int Y0) {
+    t.bw = bw;
+    t.rows = rows;
+    t.area = bw * rows;
     t.lx0 = x0 - X0;
     t.ly0 = y0 - Y0;
-    t.kx0 = x0 - ix;
-    t.ky0 = y0 - iy;
+    t.kx0 = x0 - t.f.ix;
+    t.ky0 = y0 - t.f.iy;
+}
+
+// Chord parameters (the chord walk only).
+__device__ __forceinline__ void chord_setup(TileRec& t, bool box_walk) {
     t.full = true;  // the box walk visits whole box rows: no chord parameters
     t.A = t.invA = t.det = t.P = 0.0f;
     if (!box_walk) {
@@ -681,6 +689,41 @@ __device__ __forceinline__ TileRec decode_tile_rec(const uint4* __restrict__ rec
         t.full = !(t.A > 1e-30f) || !(t.A < 1e30f) || !(fabsf(t.det) < INFINITY) || !(fabsf(t.P) < INFINITY) ||
                  !(t.invA < INFINITY);
     }
+}
+
+__device__ __forceinline__ TileRec decode_tile_rec(const uint4* __restrict__ recs, size_t j, int X0, int Y0, int X1,
+                                                   int Y1, int half, bool box_walk) {
+    TileRec t;
+    int rx, ry, pxc, pyc;
+    decode_frame(recs, j, X0, Y0, half, t, rx, ry, pxc, pyc);
+    int x0 = max(max(pxc - rx, X0), 0), x1 = min(pxc + rx, X1);
+    int y0 = max(max(pyc - ry, Y0), 0), y1 = min(pyc + ry, Y1);
+    clip_to_ellipse(t.f, x0, x1, y0, y1);
+    const bool any = x0 <= x1 && y0 <= y1;
+    set_tile_box(t, x0, y0, any ? x1 - x0 + 1 : 1, any ? y1 - y0 + 1 : 0, X0, Y0);
+    chord_setup(t, box_walk);
+    return t;
+}
+
+// A decoded record's walk length (its sort key, clamped to 255) and box in
+// one word: lx0 | ly0 << 6 | (bw - 1) << 12 | (rows - 1) << 18 | key << 24
+// (tile edge <= 64), so that a second pass over the segment's records needs
+// no window or ellipse clipping.  Key 0: nothing to walk (box left out).
+__device__ __forceinline__ uint32_t pack_tile_box(const TileRec& t, uint32_t key) {
+    if (key == 0u) return 0u;
+    return (uint32_t)t.lx0 | ((uint32_t)t.ly0 << 6) | ((uint32_t)(t.bw - 1) << 12) |
+           ((uint32_t)(t.rows - 1) << 18) | (key << 24);
+}
+
+// decode_tile_rec with the box given by pack_tile_box (key > 0): the same record.
+__device__ __forceinline__ TileRec decode_tile_rec_boxed(const uint4* __restrict__ recs, size_t j, int X0, int Y0,
+                                                         int half, uint32_t box, bool box_walk) {
+    TileRec t;
+    int rx, ry, pxc, pyc;
+    decode_frame(recs, j, X0, Y0, half, t, rx, ry, pxc, pyc);
+    const int lx0 = box & 63, ly0 = (box >> 6) & 63, bw = ((box >> 12) & 63) + 1, rows = ((box >> 18) & 63) + 1;
+    set_tile_box(t, X0 + lx0, Y0 + ly0, bw, rows, X0, Y0);
+    chord_setup(t, box_walk);
     return t;
 }
 
@@ -688,8 +731,8 @@ __device__ __forceinline__ TileRec decode_tile_rec(const uint4* __restrict__ rec
 // to 255; the key is the exact count below that), so that the 64 records a
 // wavefront walks side by side, one per lane, have nearly the same length.
 struct SegSort {
+    uint32_t box[kMaxSeg];  // pack_tile_box of every record (key in bits 24-31)
     uint16_t order[kMaxSeg];
-    uint8_t key[kMaxSeg];
     uint32_t hist[256];
     uint32_t wsum[kTileThreads / 64];
 };
@@ -703,7 +746,7 @@ __device__ __forceinline__ void sort_segment(SegSort& S, const uint4* __restrict
     for (int j = tid; j < d.z; j += kTileThreads) {
         const TileRec t = decode_tile_rec(recs, (size_t)d.y + j, X0, Y0, X1, Y1, half, box_walk);
         const uint32_t key = (uint32_t)min(chord_pixels(t), 255);
-        S.key[j] = (uint8_t)key;
+        S.box[j] = pack_tile_box(t, key);
         atomicAdd(&S.hist[key], 1u);
     }
     __syncthreads();
@@ -716,10 +759,23 @@ __device__ __forceinline__ void sort_segment(SegSort& S, const uint4* __restrict
     S.hist[tid] = excl;
     __syncthreads();
     for (int j = tid; j < d.z; j += kTileThreads) {
-        const uint32_t pos = atomicAdd(&S.hist[S.key[j]], 1u);
+        const uint32_t pos = atomicAdd(&S.hist[S.box[j] >> 24], 1u);
         S.order[pos] = (uint16_t)j;
     }
     __syncthreads();
+}
+
+// Wavefront work queue over a segment's sorted positions: chunks of 64
+// consecutive positions, handed out largest walks first (the sort is
+// ascending), so that the workgroup's four wavefronts finish together.  A
+// static round-robin split gives the last wavefront the largest records of
+// every round and leaves the others waiting at the flush barrier.
+// Returns the chunk index, or -1 when the queue is empty.
+__device__ __forceinline__ int next_chunk(int* counter, int nchunks) {
+    int c = 0;
+    if ((threadIdx.x & 63) == 0) c = atomicAdd(counter, 1);
+    c = __shfl(c, 0);
+    return c < nchunks ? nchunks - 1 - c : -1;
 }
 
 // Segment geometry: channel and tile of the segment's bin.
@@ -809,6 +865,8 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_kernel(Geom g, PhotonAr
     const int4 d = segs[s];
     const int fe = DET ? ctrl->fx_exp : 0;
     const int T = g.T, TT = T * T;
+    __shared__ int wq;  // next_chunk's counter
+    if (threadIdx.x == 0) wq = 0;
     for (int k = threadIdx.x; k < TT; k += kTileThreads) tile[k] = Acc(0);
     const SegTile st = seg_tile(g, I, d.x);
     const int half = T >> 1;
@@ -816,12 +874,18 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_kernel(Geom g, PhotonAr
     if (order_out)  // keep the walk order (record | key << 16) for the backward
         for (int pos = threadIdx.x; pos < d.z; pos += kTileThreads) {
             const uint32_t j = S.order[pos];
-            order_out[(size_t)d.y + pos] = j | ((uint32_t)S.key[j] << 16);
+            order_out[(size_t)d.y + pos] = j | ((S.box[j] >> 24) << 16);
         }
-    for (int pos = threadIdx.x; pos < d.z; pos += kTileThreads) {
+    const int nchunks = (d.z + 63) >> 6;
+    for (int ch; (ch = next_chunk(&wq, nchunks)) >= 0;) {
+        const int pos = (ch << 6) + (threadIdx.x & 63);
+        if (pos >= d.z) continue;
         const int j = S.order[pos];
-        const TileRec t = decode_tile_rec(recs, (size_t)d.y + j, st.X0, st.Y0, st.X1, st.Y1, half, g.box_walk);
-        const int n = S.key[j] < 255 ? (int)S.key[j] : chord_pixels(t);
+        const uint32_t box = S.box[j];
+        const int key = (int)(box >> 24);
+        if (key == 0) continue;
+        const TileRec t = decode_tile_rec_boxed(recs, (size_t)d.y + j, st.X0, st.Y0, half, box, g.box_walk);
+        const int n = key < 255 ? key : chord_pixels(t);
         auto walk = [&](auto& w) {
             for (int p = 0; p < n; ++p) {
                 const float dx = (float)w.kx - t.f.fx;
@@ -932,17 +996,22 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_bwd_kernel(Geom g, Imag
     }
     const int half = T >> 1;
     const uint16_t* order = nullptr;
-    const uint8_t* keys = nullptr;
+    const uint32_t* boxes = nullptr;
+    __shared__ int wq;  // next_chunk's counter
+    if (threadIdx.x == 0) wq = 0;
     if constexpr (!SORTED) {
         __shared__ SegSort S;
         sort_segment(S, recs, d, st.X0, st.Y0, st.X1, st.Y1, half, g.box_walk);
         order = S.order;
-        keys = S.key;
+        boxes = S.box;
     } else {
         __syncthreads();
     }
     const float hw = 0.5f * I.fw, hh = 0.5f * I.fh, iw = 2.0f / I.fw, ih = 2.0f / I.fh;
-    for (int pos = threadIdx.x; pos < d.z; pos += kTileThreads) {
+    const int nchunks = (d.z + 63) >> 6;
+    for (int ch; (ch = next_chunk(&wq, nchunks)) >= 0;) {
+        const int pos = (ch << 6) + (threadIdx.x & 63);
+        if (pos >= d.z) continue;
         int j, key;
         if constexpr (SORTED) {
             const uint32_t w = order_in[(size_t)d.y + pos];
@@ -950,7 +1019,7 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_bwd_kernel(Geom g, Imag
             key = (int)(w >> 16);
         } else {
             j = order[pos];
-            key = keys[j];
+            key = (int)(boxes[j] >> 24);
         }
         const TileRec t = decode_tile_rec(recs, (size_t)d.y + j, st.X0, st.Y0, st.X1, st.Y1, half, g.box_walk);
         const int n = key < 255 ? key : chord_pixels(t);
