@@ -976,8 +976,8 @@ __global__ __launch_bounds__(kTileThreads) void bin_convert_split_kernel(Geom g,
 // lane walks whole records (area-sorted), sums their seven terms in
 // registers and stores them, in the reference's units, at the record's slot.
 // ---------------------------------------------------------------------------
-// SORTED: the forward of the same binning left each segment's walk order in
-// order_in, so the segment is not sorted again.
+// SORTED: the forward of the same binning left each segment's walk order (and
+// every record's walk length) in order_in, so the segment is not sorted again.
 template <bool SORTED>
 __global__ __launch_bounds__(kTileThreads) void bin_tile_bwd_kernel(Geom g, ImageArgs I, const int4* __restrict__ segs,
                                                                     const uint4* __restrict__ recs, const Ctrl* ctrl,
@@ -1021,7 +1021,8 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_bwd_kernel(Geom g, Imag
             j = order[pos];
             key = (int)(boxes[j] >> 24);
         }
-        const TileRec t = decode_tile_rec(recs, (size_t)d.y + j, st.X0, st.Y0, st.X1, st.Y1, half, g.box_walk);
+        const size_t slot = (size_t)d.y + j;
+        const TileRec t = decode_tile_rec(recs, slot, st.X0, st.Y0, st.X1, st.Y1, half, g.box_walk);
         const int n = key < 255 ? key : chord_pixels(t);
         // Per pixel, with w = (1 - q) inside the footprint and 0 outside,
         // accumulate  m0 = sum w^2 g  and the moments of  w g  over the pixel
@@ -1063,77 +1064,67 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_bwd_kernel(Geom g, Imag
         const float sudx = c * fmaf(f.a00, m3, f.a01 * m4), sudy = c * fmaf(f.a00, m4, f.a01 * m5);
         const float svdx = c * fmaf(f.a10, m3, f.a11 * m4), svdy = c * fmaf(f.a10, m4, f.a11 * m5);
         const float gx0 = -fmaf(f.a00, su, f.a10 * sv), gx1 = -fmaf(f.a01, su, f.a11 * sv);
-        const size_t slot = (size_t)d.y + j;
         partials[2 * slot] = make_float4(kThreeOverPi * m0, gx0 * hw, gx1 * hh, sudx * iw);
         partials[2 * slot + 1] = make_float4(sudy * ih, svdx * iw, svdy * ih, 0.0f);
     }
 }
 
 // ---------------------------------------------------------------------------
-// backward 2: finalize in photon order -- sum each photon's record partials
-// (in tile order) and store coalesced.  Special photons (non-finite inputs,
-// more than kMaxRecsPerPhoton records) are left to bin_direct_kernel.
+// backward 2: finalize in photon order, one thread per photon -- sum each
+// photon's record partials (in tile order) and store coalesced.  Special
+// photons (non-finite inputs, more than kMaxRecsPerPhoton records), and every
+// photon when the records overflowed the workspace, are evaluated here by a
+// direct serial walk instead (NaN/inf propagate as in the reference).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void bin_finalize_kernel(PhotonArgs P, const Ctrl* ctrl,
-                                                           const uint8_t* __restrict__ info,
+__global__ __launch_bounds__(256) void bin_finalize_kernel(const float* __restrict__ grad, PhotonArgs P, ImageArgs I,
+                                                           const Ctrl* ctrl, const uint8_t* __restrict__ info,
                                                            const uint32_t* __restrict__ slots,
                                                            const float4* __restrict__ partials, GradArgs G) {
-    if (ctrl->overflow) return;
     const int64_t N = P.N;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride) {
-        const uint32_t cls = info[i];
-        if (cls == kSpecial) continue;
-        // all slot loads first, then all partial loads: up to 8 gathers in flight
-        uint32_t j[kMaxRecsPerPhoton];
-#pragma unroll
-        for (int k = 0; k < kMaxRecsPerPhoton; ++k) j[k] = k < (int)cls ? slots[(size_t)k * N + i] : 0u;
-        float4 p[2 * kMaxRecsPerPhoton];
-#pragma unroll
-        for (int k = 0; k < kMaxRecsPerPhoton; ++k) {
-            if (k < (int)cls) {
-                p[2 * k] = partials[2 * (size_t)j[k]];
-                p[2 * k + 1] = partials[2 * (size_t)j[k] + 1];
-            } else {
-                p[2 * k] = p[2 * k + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
-            }
-        }
-        float4 s0 = p[0], s1 = p[1];
-#pragma unroll
-        for (int k = 1; k < kMaxRecsPerPhoton; ++k) {
-            s0.x += p[2 * k].x;
-            s0.y += p[2 * k].y;
-            s0.z += p[2 * k].z;
-            s0.w += p[2 * k].w;
-            s1.x += p[2 * k + 1].x;
-            s1.y += p[2 * k + 1].y;
-            s1.z += p[2 * k + 1].z;
-        }
-        G.gE[i] = s0.x;
-        G.gx[i] = s0.y;
-        G.gx[N + i] = s0.z;
-        G.gM[i] = s0.w;
-        G.gM[N + i] = s1.x;
-        G.gM[2 * N + i] = 0.0f;
-        G.gM[3 * N + i] = s1.y;
-        G.gM[4 * N + i] = s1.z;
-        G.gM[5 * N + i] = 0.0f;
-    }
-}
-
-// backward 3: direct serial walk of the special photons, or of every photon
-// when the records overflowed the workspace
-__global__ __launch_bounds__(256) void bin_direct_kernel(const float* __restrict__ grad, PhotonArgs P, ImageArgs I,
-                                                         const Ctrl* ctrl, const uint8_t* __restrict__ info,
-                                                         GradArgs G) {
-    const bool all = ctrl->overflow != 0;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < P.N; i += stride) {
-        if (!all && info[i] != kSpecial) continue;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    // on overflow the scatter wrote no records, slots or classes
+    const uint32_t cls = ctrl->overflow ? (uint32_t)kSpecial : (uint32_t)info[i];
+    if (cls == kSpecial) {
         float a[7];
         photon_grad_direct(grad, P, I, i, /*clip_box=*/true, a);
-        store_grads(G, I, P.N, i, a);
+        store_grads(G, I, N, i, a);
+        return;
     }
+    // all slot loads first, then all partial loads: up to 8 gathers in flight
+    uint32_t j[kMaxRecsPerPhoton];
+#pragma unroll
+    for (int k = 0; k < kMaxRecsPerPhoton; ++k) j[k] = k < (int)cls ? slots[(size_t)k * N + i] : 0u;
+    float4 p[2 * kMaxRecsPerPhoton];
+#pragma unroll
+    for (int k = 0; k < kMaxRecsPerPhoton; ++k) {
+        if (k < (int)cls) {
+            p[2 * k] = partials[2 * (size_t)j[k]];
+            p[2 * k + 1] = partials[2 * (size_t)j[k] + 1];
+        } else {
+            p[2 * k] = p[2 * k + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+    }
+    float4 s0 = p[0], s1 = p[1];
+#pragma unroll
+    for (int k = 1; k < kMaxRecsPerPhoton; ++k) {
+        s0.x += p[2 * k].x;
+        s0.y += p[2 * k].y;
+        s0.z += p[2 * k].z;
+        s0.w += p[2 * k].w;
+        s1.x += p[2 * k + 1].x;
+        s1.y += p[2 * k + 1].y;
+        s1.z += p[2 * k + 1].z;
+    }
+    G.gE[i] = s0.x;
+    G.gx[i] = s0.y;
+    G.gx[N + i] = s0.z;
+    G.gM[i] = s0.w;
+    G.gM[N + i] = s1.x;
+    G.gM[2 * N + i] = 0.0f;
+    G.gM[3 * N + i] = s1.y;
+    G.gM[4 * N + i] = s1.z;
+    G.gM[5 * N + i] = 0.0f;
 }
 
 // ---------------------------------------------------------------------------
@@ -1319,13 +1310,9 @@ void launch_backward_tiles(const float* grad, const PhotonArgs& P, const ImageAr
         hipLaunchKernelGGL(bin_tile_bwd_kernel<false>, dim3(nsegs), dim3(kTileThreads), tile_lds, s, g, I,
                            (const int4*)B.segs, (const uint4*)B.recs, (const Ctrl*)B.ctrl, grad, B.partials,
                            (const uint32_t*)nullptr);
-    int64_t blocks = (P.N + 255) / 256;
-    if (blocks > 8192) blocks = 8192;
-    if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(bin_finalize_kernel, dim3((unsigned)blocks), dim3(256), 0, s, P, (const Ctrl*)B.ctrl,
+    const int64_t blocks = (P.N + 255) / 256;  // P.N > 0 here
+    hipLaunchKernelGGL(bin_finalize_kernel, dim3((unsigned)blocks), dim3(256), 0, s, grad, P, I, (const Ctrl*)B.ctrl,
                        (const uint8_t*)B.info, (const uint32_t*)B.slots, (const float4*)B.partials, G);
-    hipLaunchKernelGGL(bin_direct_kernel, dim3((unsigned)blocks), dim3(256), 0, s, grad, P, I, (const Ctrl*)B.ctrl,
-                       (const uint8_t*)B.info, G);
 }
 
 // Photon-order forward for problems the binned path cannot take.
