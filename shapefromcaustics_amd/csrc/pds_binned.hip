@@ -643,9 +643,8 @@ struct BoxWalk {
 
 // The record's frame and energy for the tile with origin (X0, Y0); also the
 // reference window (radii rx, ry about pixel (pxc, pyc)).
-__device__ __forceinline__ void decode_frame(const uint4* __restrict__ recs, size_t j, int X0, int Y0, int half,
-                                             TileRec& t, int& rx, int& ry, int& pxc, int& pyc) {
-    const uint4 q0 = recs[2 * j], q1 = recs[2 * j + 1];
+__device__ __forceinline__ void decode_frame_q(const uint4& q0, const uint4& q1, int X0, int Y0, int half, TileRec& t,
+                                               int& rx, int& ry, int& pxc, int& pyc) {
     const uint32_t pk = q0.w;
     const int ix = X0 + half + (int)(int8_t)(pk & 0xff);
     const int iy = Y0 + half + (int)(int8_t)((pk >> 8) & 0xff);
@@ -662,6 +661,11 @@ __device__ __forceinline__ void decode_frame(const uint4* __restrict__ recs, siz
     t.f.a01 = __uint_as_float(q1.y);
     t.f.a10 = __uint_as_float(q1.z);
     t.f.a11 = __uint_as_float(q1.w);
+}
+
+__device__ __forceinline__ void decode_frame(const uint4* __restrict__ recs, size_t j, int X0, int Y0, int half,
+                                             TileRec& t, int& rx, int& ry, int& pxc, int& pyc) {
+    decode_frame_q(recs[2 * j], recs[2 * j + 1], X0, Y0, half, t, rx, ry, pxc, pyc);
 }
 
 // Box fields from the box's first pixel (x0, y0), width and height (rows = 0:
@@ -691,11 +695,11 @@ __device__ __forceinline__ void chord_setup(TileRec& t, bool box_walk) {
     }
 }
 
-__device__ __forceinline__ TileRec decode_tile_rec(const uint4* __restrict__ recs, size_t j, int X0, int Y0, int X1,
-                                                   int Y1, int half, bool box_walk) {
+__device__ __forceinline__ TileRec decode_tile_rec_q(const uint4& q0, const uint4& q1, int X0, int Y0, int X1, int Y1,
+                                                     int half, bool box_walk) {
     TileRec t;
     int rx, ry, pxc, pyc;
-    decode_frame(recs, j, X0, Y0, half, t, rx, ry, pxc, pyc);
+    decode_frame_q(q0, q1, X0, Y0, half, t, rx, ry, pxc, pyc);
     int x0 = max(max(pxc - rx, X0), 0), x1 = min(pxc + rx, X1);
     int y0 = max(max(pyc - ry, Y0), 0), y1 = min(pyc + ry, Y1);
     clip_to_ellipse(t.f, x0, x1, y0, y1);
@@ -703,6 +707,11 @@ __device__ __forceinline__ TileRec decode_tile_rec(const uint4* __restrict__ rec
     set_tile_box(t, x0, y0, any ? x1 - x0 + 1 : 1, any ? y1 - y0 + 1 : 0, X0, Y0);
     chord_setup(t, box_walk);
     return t;
+}
+
+__device__ __forceinline__ TileRec decode_tile_rec(const uint4* __restrict__ recs, size_t j, int X0, int Y0, int X1,
+                                                   int Y1, int half, bool box_walk) {
+    return decode_tile_rec_q(recs[2 * j], recs[2 * j + 1], X0, Y0, X1, Y1, half, box_walk);
 }
 
 // A decoded record's walk length (its sort key, clamped to 255) and box in
@@ -743,11 +752,27 @@ __device__ __forceinline__ void sort_segment(SegSort& S, const uint4* __restrict
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     S.hist[tid] = 0;
     __syncthreads();
-    for (int j = tid; j < d.z; j += kTileThreads) {
-        const TileRec t = decode_tile_rec(recs, (size_t)d.y + j, X0, Y0, X1, Y1, half, box_walk);
-        const uint32_t key = (uint32_t)min(chord_pixels(t), 255);
-        S.box[j] = pack_tile_box(t, key);
-        atomicAdd(&S.hist[key], 1u);
+    // at most kMaxSeg / kTileThreads records per thread, all loaded before any
+    // is decoded so that the loads are in flight together
+    constexpr int R = kMaxSeg / kTileThreads;
+    uint4 q0[R], q1[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int j = tid + r * kTileThreads;
+        if (j < d.z) {
+            q0[r] = recs[2 * ((size_t)d.y + j)];
+            q1[r] = recs[2 * ((size_t)d.y + j) + 1];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int j = tid + r * kTileThreads;
+        if (j < d.z) {
+            const TileRec t = decode_tile_rec_q(q0[r], q1[r], X0, Y0, X1, Y1, half, box_walk);
+            const uint32_t key = (uint32_t)min(chord_pixels(t), 255);
+            S.box[j] = pack_tile_box(t, key);
+            atomicAdd(&S.hist[key], 1u);
+        }
     }
     __syncthreads();
     const uint32_t v = S.hist[tid];
