@@ -448,37 +448,66 @@ __global__ __launch_bounds__(kBinThreads) void bin_scatter_kernel(PhotonArgs P, 
     const int64_t lo = (int64_t)blockIdx.x * g.per_block;
     const int64_t hi = min(lo + g.per_block, P.N);
     const int half = g.T >> 1;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // Each round of records is staged in LDS and written by lane pairs, one
+    // 32-byte record per pair (lane 2s: first half, lane 2s+1: second half of
+    // lane s's record), so that the two halves of a record leave in one
+    // contiguous request instead of two scattered 16-byte ones.
+    __shared__ uint4 stage[kBinThreads / 64][2][64];
+    __shared__ uint32_t stage_slot[kBinThreads / 64][64];
     int64_t i = lo + threadIdx.x;
     // two photons of this lane in flight while one is binned
     RawPhoton r0 = {}, r1 = {};
     if (i < hi) r0 = load_photon(P, i);
     if (i + kBinThreads < hi) r1 = load_photon(P, i + kBinThreads);
-    for (; i < hi; i += kBinThreads) {
+    // wavefront-uniform rounds (the pair writes need the whole wavefront)
+    for (int64_t first = lo + (threadIdx.x & ~63); first < hi; first += kBinThreads, i += kBinThreads) {
+        const bool valid = i < hi;
         const RawPhoton now = r0;
         r0 = r1;
         if (i + 2 * kBinThreads < hi) r1 = load_photon(P, i + 2 * kBinThreads);
-        const BinPhoton b = bin_photon(now, I);
-        int n = 0;
+        BinPhoton b = {};
+        if (valid) b = bin_photon(now, I);
+        int tx0 = 0, ty0 = 0, nx = 1, nrec = 0;
         if (b.rec) {
-            const uint4 r0 = make_uint4(__float_as_uint(b.f.fx), __float_as_uint(b.f.fy), __float_as_uint(b.e), 0u);
-            const uint4 r1 = make_uint4(__float_as_uint(b.f.a00), __float_as_uint(b.f.a01), __float_as_uint(b.f.a10),
-                                        __float_as_uint(b.f.a11));
-            const int tx0 = b.x0 >> g.tshift, tx1 = b.x1 >> g.tshift;
-            const int ty0 = b.y0 >> g.tshift, ty1 = b.y1 >> g.tshift;
-            for (int ty = ty0; ty <= ty1; ++ty) {
-                for (int tx = tx0; tx <= tx1; ++tx) {
-                    const uint32_t slot = atomicAdd(&cur[(b.c * g.tiles_y + ty) * g.tiles_x + tx], 1u);
-                    uint4 q = r0;
-                    q.w = pack_rec(b.f.ix - ((tx << g.tshift) + half), b.f.iy - ((ty << g.tshift) + half), b.rx,
-                                   b.ry, b.ox, b.oy);
-                    recs[2 * (size_t)slot] = q;
-                    recs[2 * (size_t)slot + 1] = r1;
-                    if (KEEP && n < kMaxRecsPerPhoton) slots[(size_t)n * P.N + i] = slot;
-                    ++n;
+            tx0 = b.x0 >> g.tshift;
+            ty0 = b.y0 >> g.tshift;
+            nx = (b.x1 >> g.tshift) - tx0 + 1;
+            nrec = nx * ((b.y1 >> g.tshift) - ty0 + 1);
+        }
+        int kmax = nrec;
+        for (int o = 32; o > 0; o >>= 1) kmax = max(kmax, __shfl_xor(kmax, o));
+        const uint4 h0 = make_uint4(__float_as_uint(b.f.fx), __float_as_uint(b.f.fy), __float_as_uint(b.e), 0u);
+        const uint4 h1 = make_uint4(__float_as_uint(b.f.a00), __float_as_uint(b.f.a01), __float_as_uint(b.f.a10),
+                                    __float_as_uint(b.f.a11));
+        int kx = 0, ky = 0;  // this lane's current tile, relative to (tx0, ty0), row-major
+        for (int k = 0; k < kmax; ++k) {
+            const bool has = k < nrec;
+            if (has) {
+                const int tx = tx0 + kx, ty = ty0 + ky;
+                const uint32_t slot = atomicAdd(&cur[(b.c * g.tiles_y + ty) * g.tiles_x + tx], 1u);
+                uint4 q = h0;
+                q.w = pack_rec(b.f.ix - ((tx << g.tshift) + half), b.f.iy - ((ty << g.tshift) + half), b.rx, b.ry,
+                               b.ox, b.oy);
+                stage[w][0][lane] = q;
+                stage[w][1][lane] = h1;
+                stage_slot[w][lane] = slot;
+                if (KEEP && k < kMaxRecsPerPhoton) slots[(size_t)k * P.N + i] = slot;
+                if (++kx == nx) {
+                    kx = 0;
+                    ++ky;
                 }
             }
+            const uint64_t mask = __ballot(has);
+            wave_lds_fence();
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                const int src = (lane >> 1) + 32 * p, h = lane & 1;
+                if ((mask >> src) & 1ull) recs[2 * (size_t)stage_slot[w][src] + h] = stage[w][h][src];
+            }
+            wave_lds_fence();
         }
-        if (KEEP) info[i] = (b.special || n > kMaxRecsPerPhoton) ? kSpecial : (uint8_t)n;
+        if (KEEP && valid) info[i] = (b.special || nrec > kMaxRecsPerPhoton) ? kSpecial : (uint8_t)nrec;
     }
     // clear the tiles of split bins (first segment of each such bin)
     if (!clear) return;
