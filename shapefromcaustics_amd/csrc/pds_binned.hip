@@ -1062,64 +1062,80 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_bwd_kernel(Geom g, Imag
         __syncthreads();
     }
     const float hw = 0.5f * I.fw, hh = 0.5f * I.fh, iw = 2.0f / I.fw, ih = 2.0f / I.fh;
+    // partials staged per wavefront and written by lane pairs (one 32-byte
+    // request per record, as in the scatter)
+    __shared__ float4 pst[kTileThreads / 64][2][64];
+    __shared__ uint32_t pj[kTileThreads / 64][64];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int nchunks = (d.z + 63) >> 6;
     for (int ch; (ch = next_chunk(&wq, nchunks)) >= 0;) {
-        const int pos = (ch << 6) + (threadIdx.x & 63);
-        if (pos >= d.z) continue;
-        int j, key;
-        if constexpr (SORTED) {
-            const uint32_t w = order_in[(size_t)d.y + pos];
-            j = (int)(w & 0xffffu);
-            key = (int)(w >> 16);
-        } else {
-            j = order[pos];
-            key = (int)(boxes[j] >> 24);
-        }
-        const size_t slot = (size_t)d.y + j;
-        const TileRec t = decode_tile_rec(recs, slot, st.X0, st.Y0, st.X1, st.Y1, half, g.box_walk);
-        const int n = key < 255 ? key : chord_pixels(t);
-        // Per pixel, with w = (1 - q) inside the footprint and 0 outside,
-        // accumulate  m0 = sum w^2 g  and the moments of  w g  over the pixel
-        // offsets (dx, dy):  m1 = sum w g dx, m2 = sum w g dy, m3 = sum w g dx^2,
-        // m4 = sum w g dx dy, m5 = sum w g dy^2.  With s = dk / r = c w g,
-        // c = -(12/pi) E, and (u, v) = A (dx, dy), the reference's sums
-        // (photon_differentials.cu:195-204) are linear in these moments.
-        float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f, m4 = 0.f, m5 = 0.f;
-        auto walk = [&](auto& cw) {
-            for (int p = 0; p < n; ++p) {
-                const float dx = (float)cw.kx - t.f.fx, dy = cw.dy;
-                float u, v;
-                const float q = eval_q(t.f, dx, dy, u, v);
-                const float w = q < 1.0f ? 1.0f - q : 0.0f;
-                const float wg = w * gt[cw.li];
-                m0 = fmaf(w, wg, m0);
-                const float px = wg * dx, py = wg * dy;
-                m1 += px;
-                m2 += py;
-                m3 = fmaf(px, dx, m3);
-                m4 = fmaf(px, dy, m4);
-                m5 = fmaf(py, dy, m5);
-                cw.step(t, g.tshift, p + 1 < n);
+        const int pos = (ch << 6) + lane;
+        const bool act = pos < d.z;
+        if (act) {
+            int j, key;
+            if constexpr (SORTED) {
+                const uint32_t w = order_in[(size_t)d.y + pos];
+                j = (int)(w & 0xffffu);
+                key = (int)(w >> 16);
+            } else {
+                j = order[pos];
+                key = (int)(boxes[j] >> 24);
             }
-        };
-        if (g.box_walk) {
-            BoxWalk cw;
-            cw.start(t, g.tshift);
-            walk(cw);
-        } else {
-            ChordWalk cw;
-            if (n > 0) cw.start(t, g.tshift);
-            walk(cw);
+            const size_t slot = (size_t)d.y + j;
+            const TileRec t = decode_tile_rec(recs, slot, st.X0, st.Y0, st.X1, st.Y1, half, g.box_walk);
+            const int n = key < 255 ? key : chord_pixels(t);
+            // Per pixel, with w = (1 - q) inside the footprint and 0 outside,
+            // accumulate  m0 = sum w^2 g  and the moments of  w g  over the pixel
+            // offsets (dx, dy):  m1 = sum w g dx, m2 = sum w g dy, m3 = sum w g dx^2,
+            // m4 = sum w g dx dy, m5 = sum w g dy^2.  With s = dk / r = c w g,
+            // c = -(12/pi) E, and (u, v) = A (dx, dy), the reference's sums
+            // (photon_differentials.cu:195-204) are linear in these moments.
+            float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f, m4 = 0.f, m5 = 0.f;
+            auto walk = [&](auto& cw) {
+                for (int p = 0; p < n; ++p) {
+                    const float dx = (float)cw.kx - t.f.fx, dy = cw.dy;
+                    float u, v;
+                    const float q = eval_q(t.f, dx, dy, u, v);
+                    const float w = q < 1.0f ? 1.0f - q : 0.0f;
+                    const float wg = w * gt[cw.li];
+                    m0 = fmaf(w, wg, m0);
+                    const float px = wg * dx, py = wg * dy;
+                    m1 += px;
+                    m2 += py;
+                    m3 = fmaf(px, dx, m3);
+                    m4 = fmaf(px, dy, m4);
+                    m5 = fmaf(py, dy, m5);
+                    cw.step(t, g.tshift, p + 1 < n);
+                }
+            };
+            if (g.box_walk) {
+                BoxWalk cw;
+                cw.start(t, g.tshift);
+                walk(cw);
+            } else {
+                ChordWalk cw;
+                if (n > 0) cw.start(t, g.tshift);
+                walk(cw);
+            }
+            const Frame& f = t.f;
+            const float c = -kTwelveOverPi * t.e;
+            const float su = c * fmaf(f.a00, m1, f.a01 * m2);  // sum s u
+            const float sv = c * fmaf(f.a10, m1, f.a11 * m2);  // sum s v
+            const float sudx = c * fmaf(f.a00, m3, f.a01 * m4), sudy = c * fmaf(f.a00, m4, f.a01 * m5);
+            const float svdx = c * fmaf(f.a10, m3, f.a11 * m4), svdy = c * fmaf(f.a10, m4, f.a11 * m5);
+            const float gx0 = -fmaf(f.a00, su, f.a10 * sv), gx1 = -fmaf(f.a01, su, f.a11 * sv);
+            pst[wv][0][lane] = make_float4(kThreeOverPi * m0, gx0 * hw, gx1 * hh, sudx * iw);
+            pst[wv][1][lane] = make_float4(sudy * ih, svdx * iw, svdy * ih, 0.0f);
+            pj[wv][lane] = (uint32_t)j;
         }
-        const Frame& f = t.f;
-        const float c = -kTwelveOverPi * t.e;
-        const float su = c * fmaf(f.a00, m1, f.a01 * m2);  // sum s u
-        const float sv = c * fmaf(f.a10, m1, f.a11 * m2);  // sum s v
-        const float sudx = c * fmaf(f.a00, m3, f.a01 * m4), sudy = c * fmaf(f.a00, m4, f.a01 * m5);
-        const float svdx = c * fmaf(f.a10, m3, f.a11 * m4), svdy = c * fmaf(f.a10, m4, f.a11 * m5);
-        const float gx0 = -fmaf(f.a00, su, f.a10 * sv), gx1 = -fmaf(f.a01, su, f.a11 * sv);
-        partials[2 * slot] = make_float4(kThreeOverPi * m0, gx0 * hw, gx1 * hh, sudx * iw);
-        partials[2 * slot + 1] = make_float4(sudy * ih, svdx * iw, svdy * ih, 0.0f);
+        const uint64_t mask = __ballot(act);
+        wave_lds_fence();
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            const int src = (lane >> 1) + 32 * p, h = lane & 1;
+            if ((mask >> src) & 1ull) partials[2 * ((size_t)d.y + pj[wv][src]) + h] = pst[wv][h][src];
+        }
+        wave_lds_fence();
     }
 }
 
