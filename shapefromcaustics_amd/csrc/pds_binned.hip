@@ -190,11 +190,13 @@ struct RawPhoton {
     float e, cx, cy, radius, m00, m01, m02, m10, m11, m12;
 };
 
-__device__ __forceinline__ RawPhoton load_photon(const PhotonArgs& P, int64_t i) {
+// with_e = false: the energy is not read (0), for passes that only need the
+// photon's footprint (which does not depend on it)
+__device__ __forceinline__ RawPhoton load_photon(const PhotonArgs& P, int64_t i, bool with_e = true) {
     const int64_t N = P.N;
     RawPhoton r;
     r.c = P.cp[i];
-    r.e = P.E[i];
+    r.e = with_e ? P.E[i] : 0.0f;
     r.cx = P.xp[i];
     r.cy = P.xp[N + i];
     r.radius = P.radius[i];
@@ -292,12 +294,12 @@ __global__ __launch_bounds__(kBinThreads) void bin_count_kernel(PhotonArgs P, Im
     int64_t i = lo + threadIdx.x;
     // two photons of this lane in flight while one is binned
     RawPhoton r0 = {}, r1 = {};
-    if (i < hi) r0 = load_photon(P, i);
-    if (i + kBinThreads < hi) r1 = load_photon(P, i + kBinThreads);
+    if (i < hi) r0 = load_photon(P, i, g.det);
+    if (i + kBinThreads < hi) r1 = load_photon(P, i + kBinThreads, g.det);
     for (; i < hi; i += kBinThreads) {
         const RawPhoton now = r0;
         r0 = r1;
-        if (i + 2 * kBinThreads < hi) r1 = load_photon(P, i + 2 * kBinThreads);
+        if (i + 2 * kBinThreads < hi) r1 = load_photon(P, i + 2 * kBinThreads, g.det);
         const BinPhoton b = bin_photon(now, I);
         if (!b.rec) continue;
         if (b.e > 0.0f && isfinite(b.e)) emax = fmaxf(emax, b.e);
