@@ -19,7 +19,8 @@
 //                LDS accumulation, flush (store, or atomic add when split)
 //   5. convert   deterministic mode: int64 fixed point -> float for split bins
 //
-// Atomic mode accumulates float in LDS (summation order varies run to run).
+// Atomic mode accumulates in a double LDS tile (summation order varies run to
+// run; the flush rounds each pixel to float once).
 // Deterministic mode accumulates every contribution as a 64-bit fixed-point
 // integer (quantum 2^e with 2^e * 2^61 >= N max(E) 3/pi, far below FP32
 // resolution of any pixel), so the sum is exact and order-independent: the
