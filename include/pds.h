@@ -61,7 +61,7 @@ typedef enum pds_mode {
  * window (max_pixel_radius) is never altered. */
 typedef struct pds_tuning {
     int32_t tile;          /* square screen tile edge in pixels: 8, 16, 32 or 64 */
-    int32_t seg_photons;   /* photon records per tile segment (one workgroup each), <= 1024 */
+    int32_t seg_photons;   /* photon records per tile segment (one workgroup each), <= 2048 */
     int32_t block_photons; /* photons per binning workgroup */
     int32_t capacity_pct;  /* records reserved per 100 photons (default 200); more
                               records than that run the photon-order path */

@@ -34,7 +34,7 @@ import torch
 _lock = threading.Lock()
 
 TILES = (8, 16, 32, 64)
-SEGS = (256, 512, 1024)
+SEGS = (512, 1024, 2048)
 BLOCKS = (2048, 4096, 8192)
 HEADROOM = 1.25  # capacity = measured records x HEADROOM (+ 5 per 100 photons)
 

@@ -44,10 +44,11 @@ namespace {
 constexpr int kBinThreads = 512;
 constexpr int kMaxBins = 16384;
 constexpr int kMaxBinBlocks = 1024;
-constexpr int kTileThreads = 256;
-constexpr int kMaxSeg = 1024;  // records per segment (sizes the LDS sort of a segment)
+constexpr int kTileThreads = 512;
+constexpr int kMaxSeg = 2048;  // records per segment (sizes the LDS sort of a segment)
+constexpr int kSortKeys = 256;  // walk lengths 0..255 (clamped), sorted by one LDS counting sort
 constexpr int kMinBins = 256;
-constexpr int kDefaultSeg = 1024;
+constexpr int kDefaultSeg = 2048;
 constexpr double kDefaultCapacity = 2.0;  // records per photon reserved in the workspace
 constexpr int kMaxRecsPerPhoton = 4;      // backward: photons with more records are recomputed directly
 constexpr uint8_t kSpecial = 255;         // class of photons the backward recomputes directly
@@ -774,15 +775,16 @@ __device__ __forceinline__ TileRec decode_tile_rec_boxed(const uint4* __restrict
 struct SegSort {
     uint32_t box[kMaxSeg];  // pack_tile_box of every record (key in bits 24-31)
     uint16_t order[kMaxSeg];
-    uint32_t hist[256];
-    uint32_t wsum[kTileThreads / 64];
+    uint32_t hist[kSortKeys];
+    uint32_t wsum[kSortKeys / 64];
 };
 
 __device__ __forceinline__ void sort_segment(SegSort& S, const uint4* __restrict__ recs, const int4& d, int X0, int Y0,
                                              int X1, int Y1, int half, bool box_walk) {
-    static_assert(kTileThreads == 256, "one histogram bin per thread");
+    static_assert(kTileThreads >= kSortKeys && kSortKeys % 64 == 0, "one histogram bin per thread of the scan");
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-    S.hist[tid] = 0;
+    const bool scan = tid < kSortKeys;  // whole wavefronts
+    if (scan) S.hist[tid] = 0;
     __syncthreads();
     // at most kMaxSeg / kTileThreads records per thread, all loaded before any
     // is decoded so that the loads are in flight together
@@ -807,13 +809,18 @@ __device__ __forceinline__ void sort_segment(SegSort& S, const uint4* __restrict
         }
     }
     __syncthreads();
-    const uint32_t v = S.hist[tid];
-    const uint32_t incl = (uint32_t)wave_inclusive_sum((int64_t)v);
-    if (lane == 63) S.wsum[w] = incl;
+    uint32_t v = 0, incl = 0;
+    if (scan) {
+        v = S.hist[tid];
+        incl = (uint32_t)wave_inclusive_sum((int64_t)v);
+        if (lane == 63) S.wsum[w] = incl;
+    }
     __syncthreads();
-    uint32_t excl = incl - v;
-    for (int k = 0; k < w; ++k) excl += S.wsum[k];
-    S.hist[tid] = excl;
+    if (scan) {
+        uint32_t excl = incl - v;
+        for (int k = 0; k < w; ++k) excl += S.wsum[k];
+        S.hist[tid] = excl;
+    }
     __syncthreads();
     for (int j = tid; j < d.z; j += kTileThreads) {
         const uint32_t pos = atomicAdd(&S.hist[S.box[j] >> 24], 1u);
@@ -824,7 +831,7 @@ __device__ __forceinline__ void sort_segment(SegSort& S, const uint4* __restrict
 
 // Wavefront work queue over a segment's sorted positions: chunks of 64
 // consecutive positions, handed out largest walks first (the sort is
-// ascending), so that the workgroup's four wavefronts finish together.  A
+// ascending), so that the workgroup's wavefronts finish together.  A
 // static round-robin split gives the last wavefront the largest records of
 // every round and leaves the others waiting at the flush barrier.
 // Returns the chunk index, or -1 when the queue is empty.
@@ -862,7 +869,8 @@ __device__ __forceinline__ void flat_splat_all(const PhotonArgs& P, const ImageA
     WaveRecs& R = recs[w];
     R.mark[lane] = -1;
     const int64_t nbatch = (P.N + 63) >> 6;
-    for (int64_t b = (int64_t)blockIdx.x * 4 + w; b < nbatch; b += (int64_t)gridDim.x * 4) {
+    for (int64_t b = (int64_t)blockIdx.x * (kTileThreads / 64) + w; b < nbatch;
+         b += (int64_t)gridDim.x * (kTileThreads / 64)) {
         const int64_t i = (b << 6) + lane;
         FlatRec r;
         setup_flat_rec<kFwdUnit>(P, I, i, /*forward=*/true, r);
