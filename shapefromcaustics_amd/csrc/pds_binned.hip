@@ -861,7 +861,8 @@ __device__ __forceinline__ SegTile seg_tile(const Geom& g, const ImageArgs& I, i
 }
 
 // Photon-order forward of every photon (flattened walk, global atomics or
-// global int64 atomics), grid-stride over 64-photon batches; 256 threads.
+// global int64 atomics), grid-stride over 64-photon batches, one per wavefront
+// of the kTileThreads-thread workgroup.
 template <bool DET>
 __device__ __forceinline__ void flat_splat_all(const PhotonArgs& P, const ImageArgs& I, int fe, WaveRecs* recs,
                                                float* __restrict__ out, unsigned long long* __restrict__ acc64) {
@@ -1363,7 +1364,7 @@ void launch_build(const PhotonArgs& P, const ImageArgs& I, const Geom& g, const 
 void launch_forward_tiles(const PhotonArgs& P, const ImageArgs& I, const Geom& g, const Buffers& B, float* out,
                           bool keep, hipStream_t s) {
     const unsigned nsegs = (unsigned)g.max_segments;
-    // the tile accumulator, or (photon-order path) four waves' walk buffers
+    // the tile accumulator, or (photon-order path) the wavefronts' walk buffers
     const size_t tile_lds = std::max((size_t)g.T * g.T * 8, sizeof(WaveRecs) * (kTileThreads / 64));
     if (g.det) {
         hipLaunchKernelGGL(bin_tile_kernel<true>, dim3(nsegs), dim3(kTileThreads), tile_lds, s, g, P, I,
