@@ -85,7 +85,8 @@ def bench_landweber(args):
     through a 256^2 height field (32 Gaussian bumps, 1 mm base, n = 1.5) to a
     sensor 50 mm below, splatted into 1024^2 (cap 12); loss 1/2 |I - T|^2
     against the image of a differently seeded field; PDS adjoint, path
-    adjoint (recomputed) and the masked shrinkage + volume update."""
+    adjoint (replaying the forward's ray hits) and the masked shrinkage +
+    volume update."""
     import numpy as np
     import torch
 
@@ -141,14 +142,14 @@ def bench_landweber(args):
     h = (d + base).contiguous()
     e[0].record()
     n = HF.normals(h)
-    Ep, xp, Mp, cp, radius = HF.simulate(h, sc, em, n)
+    Ep, xp, Mp, cp, radius, hits = HF.simulate(h, sc, em, n, keep_hits=True)
     e[1].record()
     img, bins = _forward(Ep, xp, Mp, cp, radius, size, cap, "auto", None, keep_bins=True)
     resid, lossv = HF.residual(img, target)
     e[2].record()
     gE, gx, gM = _backward(resid, Ep, xp, Mp, cp, radius, cap, "auto", None, bins=bins)
     e[3].record()
-    gh = HF.adjoint(h, sc, em, gE, gx, gM, n)
+    gh = HF.adjoint(h, sc, em, gE, gx, gM, n, hits=hits)
     e[4].record()
     HF.landweber_update(d, gh, mask, params)
     e[5].record()

@@ -109,6 +109,23 @@ int hf_adjoint_f32(const float* h, const float* normals, const hf_scene* scene, 
                    const float* grad_Ep, const float* grad_xp, const float* grad_Mp, float* grad_h,
                    void* workspace, size_t workspace_bytes, void* stream);
 
+/* The same pair sharing the forward's ray hits (no reference counterpart; the
+ * reference's backward re-runs its OptiX trace through torch autograd,
+ * model/caustics.py:242-291): hf_simulate_hits_f32 also records, for every
+ * photon that reaches the sensor, its first 4 hits (t, barycentrics,
+ * object, primitive) in `hits` (hf_hits_bytes), and hf_adjoint_hits_f32
+ * replays those paths without tracing them again (paths with more bounces
+ * are traced again).  Every photon's gradient terms are those of
+ * hf_adjoint_f32 bit for bit; dL/dh differs only by the (run-dependent)
+ * order of the float atomic sums. */
+size_t hf_hits_bytes(const hf_scene* scene, const hf_emitter* emitter);
+int hf_simulate_hits_f32(const float* h, const float* normals, const hf_scene* scene, const hf_emitter* emitter,
+                         float* Ep, float* xp, float* Mp, int64_t* cp, float* radius, void* workspace,
+                         size_t workspace_bytes, void* hits, size_t hits_bytes, void* stream);
+int hf_adjoint_hits_f32(const float* h, const float* normals, const hf_scene* scene, const hf_emitter* emitter,
+                        const float* grad_Ep, const float* grad_xp, const float* grad_Mp, float* grad_h,
+                        void* workspace, size_t workspace_bytes, const void* hits, size_t hits_bytes, void* stream);
+
 /* Residual and loss of the Landweber data term (hyperparameter_helper.py:29-30):
  * resid = img - target, loss[0] = 1/2 sum resid^2 (device scalar; FP32
  * per-block partials summed in a fixed order). */

@@ -300,17 +300,38 @@ __device__ __forceinline__ void write_dropped(const OutArgs& O, int64_t i) {
     O.cp[i] = -1;
 }
 
+// Hit records for the adjoint: the first kHitSlots hits of every photon that
+// reaches the sensor, [kHitSlots][N] float4 (t, b1, b2, obj << 30 | prim), and
+// per photon the number recorded: 0 = dropped (no gradient), kHitsRetrace =
+// more bounces than slots (the adjoint traces that path again).
+constexpr int kHitSlots = 4;
+constexpr uint8_t kHitsRetrace = 255;
+
+__device__ __forceinline__ float4 pack_hit(const Hit& h) {
+    return make_float4(h.t, h.b1, h.b2, __int_as_float((int)(((uint32_t)h.obj << 30) | (uint32_t)h.prim)));
+}
+
+__device__ __forceinline__ Hit unpack_hit(float4 v) {
+    const uint32_t op = (uint32_t)__float_as_int(v.w);
+    return Hit{v.x, v.y, v.z, (int)(op >> 30), (int)(op & 0x3fffffffu)};
+}
+
+// hits, nhit: null, or the hit records to write
 __global__ __launch_bounds__(256) void simulate_kernel(SceneArgs S, const float* __restrict__ hmax, EmitArgs E,
-                                                       OutArgs O) {
+                                                       OutArgs O, float4* __restrict__ hits,
+                                                       uint8_t* __restrict__ nhit) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= O.N) return;
     S.hmax = hmax[0];
     int ch;
     PathState s = emit(E, i, ch);
     const float ei = E.ior[ch], eo = 1.0f;
+    int nb = 0;
     for (int bounce = 0; bounce < kMaxBounces; ++bounce) {
         const Hit hit = trace(S, s.o, s.d);
         if (hit.obj < 0) break;  // miss: advance_path drops it
+        if (hits && nb < kHitSlots) hits[(int64_t)nb * O.N + i] = pack_hit(hit);
+        ++nb;
         const float t = hit.t;
         const V3 p = s.o + t * s.d;
         const TriVerts T = tri_verts(S, hit.obj, hit.prim);
@@ -333,6 +354,7 @@ __global__ __launch_bounds__(256) void simulate_kernel(SceneArgs S, const float*
             O.Mp[5 * N + i] = sp.M1.z;
             O.radius[i] = sp.radius;
             O.cp[i] = ch;
+            if (nhit) nhit[i] = nb <= kHitSlots ? (uint8_t)nb : kHitsRetrace;
             return;
         }
         const Refr R = refr_setup(s.d, ns, eo, ei);
@@ -342,6 +364,7 @@ __global__ __launch_bounds__(256) void simulate_kernel(SceneArgs S, const float*
         s.o = p;
     }
     write_dropped(O, i);
+    if (nhit) nhit[i] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -352,6 +375,27 @@ struct Tape {
     float t, b1, b2;
     int obj, prim;
 };
+
+// One bounce of the forward path after its hit: move to the hit point,
+// transfer the differentials and refract -- the update of simulate_kernel,
+// operation for operation, so a replay from recorded hits is bit-identical.
+__device__ __forceinline__ void advance(const SceneArgs& S, PathState& s, const Hit& hit, float eo, float ei) {
+    const V3 p = s.o + hit.t * s.d;
+    const TriVerts T = tri_verts(S, hit.obj, hit.prim);
+    const V3 ng = geometric_normal(T);
+    const V3 ns = normalize(shading_raw(T, hit.b1, hit.b2));
+    const V3 Du = transfer1(ng, s.d, hit.t, s.Du, s.Dt);
+    const V3 Dv = transfer1(ng, s.d, hit.t, s.Dv, s.Dp);
+    const Refr R = refr_setup(s.d, ns, eo, ei);
+    refract_apply(R, s.d, s.Dt, s.Dp);
+    s.Du = Du;
+    s.Dv = Dv;
+    s.o = p;
+}
+
+__device__ __forceinline__ Tape tape_of(const PathState& s, const Hit& h) {
+    return Tape{s.d, s.Du, s.Dv, s.Dt, s.Dp, h.t, h.b1, h.b2, h.obj, h.prim};
+}
 
 __device__ __forceinline__ void atomic_add3(float* p, V3 v) {
     unsafeAtomicAdd(p, v.x);
@@ -412,54 +456,60 @@ struct PatchAcc {
 
 __device__ __forceinline__ void adjoint_photon(const SceneArgs& S, const EmitArgs& E, int64_t i, int64_t N,
                                                const float* __restrict__ gEp, const float* __restrict__ gxp,
-                                               const float* __restrict__ gMp, const PatchAcc& A) {
+                                               const float* __restrict__ gMp, const PatchAcc& A,
+                                               const float4* __restrict__ hits, const uint8_t* __restrict__ nhit) {
+    // with the forward's hit records the path is replayed without tracing and
+    // without a tape: the state at a bounce is advanced from the emitted one
+    const int m = nhit ? (int)nhit[i] : (int)kHitsRetrace;
+    if (m == 0) return;  // dropped in the forward: no gradient
+    const bool replay = m != kHitsRetrace;
     int ch;
-    PathState s = emit(E, i, ch);
+    const PathState s0 = emit(E, i, ch);
     const float ei = E.ior[ch], eo = 1.0f;
-    Tape tape[kMaxBounces];
+    auto hit_at = [&](int b) { return unpack_hit(hits[(int64_t)b * N + i]); };
+    Tape tape[kMaxBounces];  // traced paths only
     int nb = 0;
-    bool reached = false;
-    // forward, recording
-    for (int bounce = 0; bounce < kMaxBounces; ++bounce) {
-        const Hit hit = trace(S, s.o, s.d);
-        if (hit.obj < 0) break;
-        Tape& tp = tape[nb++];
-        tp.d = s.d;
-        tp.Du = s.Du;
-        tp.Dv = s.Dv;
-        tp.Dt = s.Dt;
-        tp.Dp = s.Dp;
-        tp.t = hit.t;
-        tp.b1 = hit.b1;
-        tp.b2 = hit.b2;
-        tp.obj = hit.obj;
-        tp.prim = hit.prim;
-        const float t = hit.t;
-        const V3 p = s.o + t * s.d;
-        const TriVerts T = tri_verts(S, hit.obj, hit.prim);
-        const V3 ng = geometric_normal(T);
-        const V3 ns = normalize(shading_raw(T, hit.b1, hit.b2));
-        const V3 Du = transfer1(ng, s.d, t, s.Du, s.Dt);
-        const V3 Dv = transfer1(ng, s.d, t, s.Dv, s.Dp);
-        if (hit.obj == 1) {
-            const Splat sp = splat_inputs(p, Du, Dv, ns, E.flux);
-            reached = sp.good;
-            break;
+    PathState s = s0;
+    if (replay) {
+        nb = m;  // the last recorded hit is the ground
+        for (int b = 0; b + 1 < m; ++b) advance(S, s, hit_at(b), eo, ei);
+    } else {
+        bool reached = false;
+        // forward, recording
+        for (int bounce = 0; bounce < kMaxBounces; ++bounce) {
+            const Hit hit = trace(S, s.o, s.d);
+            if (hit.obj < 0) break;
+            tape[nb++] = tape_of(s, hit);
+            if (hit.obj == 1) {
+                const V3 p = s.o + hit.t * s.d;
+                const TriVerts T = tri_verts(S, hit.obj, hit.prim);
+                const V3 ng = geometric_normal(T);
+                const V3 ns = normalize(shading_raw(T, hit.b1, hit.b2));
+                const V3 Du = transfer1(ng, s.d, hit.t, s.Du, s.Dt);
+                const V3 Dv = transfer1(ng, s.d, hit.t, s.Dv, s.Dp);
+                const Splat sp = splat_inputs(p, Du, Dv, ns, E.flux);
+                reached = sp.good;
+                break;
+            }
+            advance(S, s, hit, eo, ei);
         }
-        const Refr R = refr_setup(s.d, ns, eo, ei);
-        refract_apply(R, s.d, s.Dt, s.Dp);
-        s.Du = Du;
-        s.Dv = Dv;
-        s.o = p;
+        if (!reached) return;
     }
-    if (!reached) return;
+    // the path at bounce k (the tape, or a replay of the recorded hits)
+    auto bounce_at = [&](int k) -> Tape {
+        if (!replay) return tape[k];
+        if (k == nb - 1) return tape_of(s, hit_at(k));
+        PathState r = s0;
+        for (int b = 0; b < k; ++b) advance(S, r, hit_at(b), eo, ei);
+        return tape_of(r, hit_at(k));
+    };
     // upstream gradients of the splat inputs
     const float gE = gEp[i];
     const float gx0 = gxp[i], gx1 = gxp[N + i];
     const V3 gM0 = v3(gMp[i], gMp[N + i], gMp[2 * N + i]);
     const V3 gM1 = v3(gMp[3 * N + i], gMp[4 * N + i], gMp[5 * N + i]);
     // ---- last bounce: ground plane (constant geometry)
-    const Tape& L = tape[nb - 1];
+    const Tape L = bounce_at(nb - 1);
     V3 gp = v3(gx0, gx1, 0.0f), gd, gDu, gDv, gDt, gDp;
     {
         const TriVerts T = tri_verts(S, L.obj, L.prim);
@@ -491,7 +541,7 @@ __device__ __forceinline__ void adjoint_photon(const SceneArgs& S, const EmitArg
     }
     // ---- earlier bounces, newest first: gradients arrive for (o' = p, d', Du', Dv', Dt', Dp')
     for (int k = nb - 2; k >= 0; --k) {
-        const Tape& B = tape[k];
+        const Tape B = bounce_at(k);
         const TriVerts T = tri_verts(S, B.obj, B.prim);
         const V3 graw = cross(T.v[1] - T.v[0], T.v[2] - T.v[0]);
         const float glen = norm(graw);
@@ -567,7 +617,9 @@ __device__ __forceinline__ void adjoint_photon(const SceneArgs& S, const EmitArg
 __global__ __launch_bounds__(256) void adjoint_kernel(SceneArgs S, const float* __restrict__ hmax, EmitArgs E,
                                                       int64_t N, int tiles_x, const float* __restrict__ gEp,
                                                       const float* __restrict__ gxp, const float* __restrict__ gMp,
-                                                      float* __restrict__ grad_n, float* __restrict__ grad_z) {
+                                                      float* __restrict__ grad_n, float* __restrict__ grad_z,
+                                                      const float4* __restrict__ hits,
+                                                      const uint8_t* __restrict__ nhit) {
     __shared__ float pn[kPatch * kPatch * 3];
     __shared__ float pz[kPatch * kPatch];
     S.hmax = hmax[0];
@@ -590,7 +642,7 @@ __global__ __launch_bounds__(256) void adjoint_kernel(SceneArgs S, const float* 
     __syncthreads();
     if (col < E.n_side && row < E.n_side) {
         const int64_t i = ((int64_t)row * E.n_side + col) * E.channels + ch;
-        adjoint_photon(S, E, i, N, gEp, gxp, gMp, A);
+        adjoint_photon(S, E, i, N, gEp, gxp, gMp, A, hits, nhit);
     }
     __syncthreads();
     for (int k = threadIdx.x; k < kPatch * kPatch; k += 256) {
@@ -802,6 +854,81 @@ int hip_status(const char* what) {
     return PDS_OK;
 }
 
+struct HitsLayout {
+    size_t recs, count, end;
+};
+
+HitsLayout hits_layout(int64_t N) {
+    HitsLayout L;
+    L.recs = 0;
+    L.count = (sizeof(float4) * kHitSlots * (size_t)N + 255) & ~(size_t)255;
+    L.end = L.count + (size_t)N;
+    return L;
+}
+
+int simulate_impl(const float* h, const float* normals, const hf_scene* sc, const hf_emitter* em, float* Ep,
+                  float* xp, float* Mp, int64_t* cp, float* radius, void* ws, size_t ws_bytes, void* hits,
+                  size_t hits_bytes, bool want_hits, void* stream) {
+    g_err[0] = '\0';
+    int rc = check_scene(sc, h, normals);
+    if (rc) return rc;
+    if ((rc = check_emitter(em))) return rc;
+    if (!Ep || !xp || !Mp || !cp || !radius) return fail(PDS_ERR_ARGUMENT, "null output buffer");
+    const WsLayout L = ws_layout(sc->R);
+    if (!ws || ws_bytes < L.end) return fail(PDS_ERR_WORKSPACE, "workspace smaller than hf_workspace_bytes(R)");
+    const int64_t N = (int64_t)em->n_side * em->n_side * sc->channels;
+    const HitsLayout HL = hits_layout(N);
+    if (want_hits && (!hits || hits_bytes < HL.end))
+        return fail(PDS_ERR_WORKSPACE, "hit buffer smaller than hf_hits_bytes");
+    float* hmax = (float*)((char*)ws + L.hmax);
+    OutArgs O{Ep, xp, Mp, radius, cp, N};
+    float4* hrec = want_hits ? (float4*)((char*)hits + HL.recs) : nullptr;
+    uint8_t* hcnt = want_hits ? (uint8_t*)hits + HL.count : nullptr;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(hmax_kernel, dim3(1), dim3(1024), 0, s, h, (int64_t)sc->R * sc->R, hmax);
+    hipLaunchKernelGGL(simulate_kernel, dim3(blocks_for(N)), dim3(256), 0, s, scene_args(h, normals, sc, hmax),
+                       (const float*)hmax, emit_args(sc, em), O, hrec, hcnt);
+    return hip_status("hf_simulate");
+}
+
+int adjoint_impl(const float* h, const float* normals, const hf_scene* sc, const hf_emitter* em,
+                 const float* grad_Ep, const float* grad_xp, const float* grad_Mp, float* grad_h, void* ws,
+                 size_t ws_bytes, const void* hits, size_t hits_bytes, bool want_hits, void* stream) {
+    g_err[0] = '\0';
+    int rc = check_scene(sc, h, normals);
+    if (rc) return rc;
+    if ((rc = check_emitter(em))) return rc;
+    if (!grad_Ep || !grad_xp || !grad_Mp || !grad_h) return fail(PDS_ERR_ARGUMENT, "null gradient buffer");
+    const WsLayout L = ws_layout(sc->R);
+    if (!ws || ws_bytes < L.end) return fail(PDS_ERR_WORKSPACE, "workspace smaller than hf_workspace_bytes(R)");
+    const int64_t N = (int64_t)em->n_side * em->n_side * sc->channels;
+    const HitsLayout HL = hits_layout(N);
+    if (want_hits && (!hits || hits_bytes < HL.end))
+        return fail(PDS_ERR_WORKSPACE, "hit buffer smaller than hf_hits_bytes");
+    char* b = (char*)ws;
+    float* hmax = (float*)(b + L.hmax);
+    float* gn = (float*)(b + L.grad_n);
+    float* gz = (float*)(b + L.grad_z);
+    float* gg = (float*)(b + L.grad_g);
+    const int R = sc->R;
+    const int64_t RR = (int64_t)R * R;
+    const float s_sobel = (float)(-1.0 / (8.0 * (2.0 / (double)(R - 1))));
+    const float4* hrec = want_hits ? (const float4*)((const char*)hits + HL.recs) : nullptr;
+    const uint8_t* hcnt = want_hits ? (const uint8_t*)hits + HL.count : nullptr;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(zero_kernel, dim3(blocks_for(3 * RR)), dim3(256), 0, s, gn, 3 * RR);
+    hipLaunchKernelGGL(zero_kernel, dim3(blocks_for(RR)), dim3(256), 0, s, gz, RR);
+    hipLaunchKernelGGL(hmax_kernel, dim3(1), dim3(1024), 0, s, h, RR, hmax);
+    const int tiles_x = (em->n_side + 15) / 16;
+    hipLaunchKernelGGL(adjoint_kernel, dim3((unsigned)(tiles_x * tiles_x), (unsigned)sc->channels), dim3(256), 0, s,
+                       scene_args(h, normals, sc, hmax), (const float*)hmax, emit_args(sc, em), N, tiles_x, grad_Ep,
+                       grad_xp, grad_Mp, gn, gz, hrec, hcnt);
+    hipLaunchKernelGGL(normals_adj_kernel, dim3(blocks_for(RR)), dim3(256), 0, s, h, R, s_sobel, (const float*)gn, gg);
+    hipLaunchKernelGGL(grad_h_kernel, dim3(blocks_for(RR)), dim3(256), 0, s, R, (const float*)gg, (const float*)gz,
+                       grad_h);
+    return hip_status("hf_adjoint");
+}
+
 }  // namespace
 
 }  // namespace hf
@@ -842,56 +969,35 @@ int hf_trace_f32(const float* h, const float* normals, const hf_scene* sc, const
     return hip_status("hf_trace");
 }
 
+size_t hf_hits_bytes(const hf_scene* sc, const hf_emitter* em) {
+    if (!sc || !em || sc->channels < 1 || em->n_side < 1) return 0;
+    const int64_t N = (int64_t)em->n_side * em->n_side * sc->channels;
+    return hits_layout(N).end;
+}
+
 int hf_simulate_f32(const float* h, const float* normals, const hf_scene* sc, const hf_emitter* em, float* Ep,
                     float* xp, float* Mp, int64_t* cp, float* radius, void* ws, size_t ws_bytes, void* stream) {
-    g_err[0] = '\0';
-    int rc = check_scene(sc, h, normals);
-    if (rc) return rc;
-    if ((rc = check_emitter(em))) return rc;
-    if (!Ep || !xp || !Mp || !cp || !radius) return fail(PDS_ERR_ARGUMENT, "null output buffer");
-    const WsLayout L = ws_layout(sc->R);
-    if (!ws || ws_bytes < L.end) return fail(PDS_ERR_WORKSPACE, "workspace smaller than hf_workspace_bytes(R)");
-    float* hmax = (float*)((char*)ws + L.hmax);
-    const int64_t N = (int64_t)em->n_side * em->n_side * sc->channels;
-    OutArgs O{Ep, xp, Mp, radius, cp, N};
-    hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(hmax_kernel, dim3(1), dim3(1024), 0, s, h, (int64_t)sc->R * sc->R, hmax);
-    hipLaunchKernelGGL(simulate_kernel, dim3(blocks_for(N)), dim3(256), 0, s, scene_args(h, normals, sc, hmax),
-                       (const float*)hmax, emit_args(sc, em), O);
-    return hip_status("hf_simulate");
+    return simulate_impl(h, normals, sc, em, Ep, xp, Mp, cp, radius, ws, ws_bytes, nullptr, 0, false, stream);
+}
+
+int hf_simulate_hits_f32(const float* h, const float* normals, const hf_scene* sc, const hf_emitter* em, float* Ep,
+                         float* xp, float* Mp, int64_t* cp, float* radius, void* ws, size_t ws_bytes, void* hits,
+                         size_t hits_bytes, void* stream) {
+    return simulate_impl(h, normals, sc, em, Ep, xp, Mp, cp, radius, ws, ws_bytes, hits, hits_bytes, true, stream);
 }
 
 int hf_adjoint_f32(const float* h, const float* normals, const hf_scene* sc, const hf_emitter* em,
                    const float* grad_Ep, const float* grad_xp, const float* grad_Mp, float* grad_h, void* ws,
                    size_t ws_bytes, void* stream) {
-    g_err[0] = '\0';
-    int rc = check_scene(sc, h, normals);
-    if (rc) return rc;
-    if ((rc = check_emitter(em))) return rc;
-    if (!grad_Ep || !grad_xp || !grad_Mp || !grad_h) return fail(PDS_ERR_ARGUMENT, "null gradient buffer");
-    const WsLayout L = ws_layout(sc->R);
-    if (!ws || ws_bytes < L.end) return fail(PDS_ERR_WORKSPACE, "workspace smaller than hf_workspace_bytes(R)");
-    char* b = (char*)ws;
-    float* hmax = (float*)(b + L.hmax);
-    float* gn = (float*)(b + L.grad_n);
-    float* gz = (float*)(b + L.grad_z);
-    float* gg = (float*)(b + L.grad_g);
-    const int R = sc->R;
-    const int64_t RR = (int64_t)R * R;
-    const int64_t N = (int64_t)em->n_side * em->n_side * sc->channels;
-    const float s_sobel = (float)(-1.0 / (8.0 * (2.0 / (double)(R - 1))));
-    hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(zero_kernel, dim3(blocks_for(3 * RR)), dim3(256), 0, s, gn, 3 * RR);
-    hipLaunchKernelGGL(zero_kernel, dim3(blocks_for(RR)), dim3(256), 0, s, gz, RR);
-    hipLaunchKernelGGL(hmax_kernel, dim3(1), dim3(1024), 0, s, h, RR, hmax);
-    const int tiles_x = (em->n_side + 15) / 16;
-    hipLaunchKernelGGL(adjoint_kernel, dim3((unsigned)(tiles_x * tiles_x), (unsigned)sc->channels), dim3(256), 0, s,
-                       scene_args(h, normals, sc, hmax), (const float*)hmax, emit_args(sc, em), N, tiles_x, grad_Ep,
-                       grad_xp, grad_Mp, gn, gz);
-    hipLaunchKernelGGL(normals_adj_kernel, dim3(blocks_for(RR)), dim3(256), 0, s, h, R, s_sobel, (const float*)gn, gg);
-    hipLaunchKernelGGL(grad_h_kernel, dim3(blocks_for(RR)), dim3(256), 0, s, R, (const float*)gg, (const float*)gz,
-                       grad_h);
-    return hip_status("hf_adjoint");
+    return adjoint_impl(h, normals, sc, em, grad_Ep, grad_xp, grad_Mp, grad_h, ws, ws_bytes, nullptr, 0, false,
+                        stream);
+}
+
+int hf_adjoint_hits_f32(const float* h, const float* normals, const hf_scene* sc, const hf_emitter* em,
+                        const float* grad_Ep, const float* grad_xp, const float* grad_Mp, float* grad_h, void* ws,
+                        size_t ws_bytes, const void* hits, size_t hits_bytes, void* stream) {
+    return adjoint_impl(h, normals, sc, em, grad_Ep, grad_xp, grad_Mp, grad_h, ws, ws_bytes, hits, hits_bytes, true,
+                        stream);
 }
 
 size_t hf_residual_workspace_bytes(int64_t n) { return n < 0 ? 0 : sizeof(float) * kRedBlocks; }

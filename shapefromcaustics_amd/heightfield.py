@@ -55,6 +55,9 @@ SIGNATURES = {
     "hf_trace_f32": ([_vp, _vp, _SP, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _sz, _vp], ctypes.c_int),
     "hf_simulate_f32": ([_vp, _vp, _SP, _EP, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp], ctypes.c_int),
     "hf_adjoint_f32": ([_vp, _vp, _SP, _EP, _vp, _vp, _vp, _vp, _vp, _sz, _vp], ctypes.c_int),
+    "hf_hits_bytes": ([_SP, _EP], _sz),
+    "hf_simulate_hits_f32": ([_vp, _vp, _SP, _EP, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp, _sz, _vp], ctypes.c_int),
+    "hf_adjoint_hits_f32": ([_vp, _vp, _SP, _EP, _vp, _vp, _vp, _vp, _vp, _sz, _vp, _sz, _vp], ctypes.c_int),
     "hf_residual_workspace_bytes": ([_i64], _sz),
     "hf_residual_f32": ([_vp, _vp, _i64, _vp, _vp, _vp, _sz, _vp], ctypes.c_int),
     "hf_landweber_workspace_bytes": ([_i32], _sz),
@@ -207,8 +210,12 @@ def trace_rays(h: torch.Tensor, scene: Scene, origins: torch.Tensor, dirs: torch
     return t, uv, obj, prim
 
 
-def simulate(h: torch.Tensor, scene: Scene, emitter: ParallelEmitter, n: tp.Optional[torch.Tensor] = None):
-    """Splat inputs (Ep, xp, Mp, cp, radius) of every emitted photon; dropped photons have cp = -1."""
+def simulate(h: torch.Tensor, scene: Scene, emitter: ParallelEmitter, n: tp.Optional[torch.Tensor] = None,
+             keep_hits: bool = False):
+    """Splat inputs (Ep, xp, Mp, cp, radius) of every emitted photon; dropped photons have cp = -1.
+
+    keep_hits: also return the photons' ray hits (a byte tensor), which
+    :func:`adjoint` replays instead of tracing every path again."""
     _check_h(h, scene.R)
     dev = h.device
     n = normals(h) if n is None else n
@@ -218,23 +225,35 @@ def simulate(h: torch.Tensor, scene: Scene, emitter: ParallelEmitter, n: tp.Opti
     cp, radius = torch.empty(1, N, dtype=torch.int64, device=dev), torch.empty(N, **f32)
     ws = _ws(scene.R, dev)
     sc, em = scene.struct(), _emitter_struct(emitter, scene)
-    _check(lib().hf_simulate_f32(_p(h), _p(n), ctypes.byref(sc), ctypes.byref(em), _p(Ep), _p(xp), _p(Mp), _p(cp),
-                                 _p(radius), _p(ws), ws.numel(), _stream(dev)), "hf_simulate_f32")
-    return Ep, xp, Mp, cp, radius
+    if not keep_hits:
+        _check(lib().hf_simulate_f32(_p(h), _p(n), ctypes.byref(sc), ctypes.byref(em), _p(Ep), _p(xp), _p(Mp),
+                                     _p(cp), _p(radius), _p(ws), ws.numel(), _stream(dev)), "hf_simulate_f32")
+        return Ep, xp, Mp, cp, radius
+    hits = torch.empty(lib().hf_hits_bytes(ctypes.byref(sc), ctypes.byref(em)), dtype=torch.uint8, device=dev)
+    _check(lib().hf_simulate_hits_f32(_p(h), _p(n), ctypes.byref(sc), ctypes.byref(em), _p(Ep), _p(xp), _p(Mp),
+                                      _p(cp), _p(radius), _p(ws), ws.numel(), _p(hits), hits.numel(), _stream(dev)),
+           "hf_simulate_hits_f32")
+    return Ep, xp, Mp, cp, radius, hits
 
 
 def adjoint(h: torch.Tensor, scene: Scene, emitter: ParallelEmitter, gE, gx, gM,
-            n: tp.Optional[torch.Tensor] = None) -> torch.Tensor:
-    """dL/dh from dL/d(Ep, xp, Mp) of :func:`simulate`'s photons."""
+            n: tp.Optional[torch.Tensor] = None, hits: tp.Optional[torch.Tensor] = None) -> torch.Tensor:
+    """dL/dh from dL/d(Ep, xp, Mp) of :func:`simulate`'s photons (hits: the
+    forward's ``keep_hits`` output for the same h, scene and emitter)."""
     _check_h(h, scene.R)
     dev = h.device
     n = normals(h) if n is None else n
     out = torch.empty_like(h)
     ws = _ws(scene.R, dev)
     sc, em = scene.struct(), _emitter_struct(emitter, scene)
-    _check(lib().hf_adjoint_f32(_p(h), _p(n), ctypes.byref(sc), ctypes.byref(em), _p(gE.contiguous()),
-                                _p(gx.contiguous()), _p(gM.contiguous()), _p(out), _p(ws), ws.numel(),
-                                _stream(dev)), "hf_adjoint_f32")
+    if hits is None:
+        _check(lib().hf_adjoint_f32(_p(h), _p(n), ctypes.byref(sc), ctypes.byref(em), _p(gE.contiguous()),
+                                    _p(gx.contiguous()), _p(gM.contiguous()), _p(out), _p(ws), ws.numel(),
+                                    _stream(dev)), "hf_adjoint_f32")
+    else:
+        _check(lib().hf_adjoint_hits_f32(_p(h), _p(n), ctypes.byref(sc), ctypes.byref(em), _p(gE.contiguous()),
+                                         _p(gx.contiguous()), _p(gM.contiguous()), _p(out), _p(ws), ws.numel(),
+                                         _p(hits), hits.numel(), _stream(dev)), "hf_adjoint_hits_f32")
     return out
 
 
@@ -244,11 +263,13 @@ class CausticImage(torch.autograd.Function):
     @staticmethod
     def forward(ctx, h, scene: Scene, emitter: ParallelEmitter, size, max_pixel_radius: int, mode="auto"):
         n = normals(h)
-        Ep, xp, Mp, cp, radius = simulate(h, scene, emitter, n)
-        img, bins = _forward(Ep, xp, Mp, cp, radius, size, max_pixel_radius, mode, None,
-                             keep_bins=ctx.needs_input_grad[0])
+        grad = ctx.needs_input_grad[0]
+        out = simulate(h, scene, emitter, n, keep_hits=grad)
+        Ep, xp, Mp, cp, radius = out[:5]
+        img, bins = _forward(Ep, xp, Mp, cp, radius, size, max_pixel_radius, mode, None, keep_bins=grad)
         ctx.save_for_backward(h, n, Ep, xp, Mp, cp, radius)
         ctx.scene, ctx.emitter, ctx.cap, ctx.mode, ctx.bins = scene, emitter, max_pixel_radius, mode, bins
+        ctx.hits = out[5] if grad else None
         return img
 
     @staticmethod
@@ -256,7 +277,8 @@ class CausticImage(torch.autograd.Function):
         h, n, Ep, xp, Mp, cp, radius = ctx.saved_tensors
         gE, gx, gM = _backward(grad_img.contiguous(), Ep, xp, Mp, cp, radius, ctx.cap, ctx.mode, None, bins=ctx.bins)
         ctx.bins = None
-        gh = adjoint(h, ctx.scene, ctx.emitter, gE, gx, gM, n)
+        gh = adjoint(h, ctx.scene, ctx.emitter, gE, gx, gM, n, hits=ctx.hits)
+        ctx.hits = None
         return gh, None, None, None, None, None
 
 
@@ -307,10 +329,10 @@ def landweber_step(d: torch.Tensor, base: float, target: torch.Tensor, scene: Sc
     Returns (loss device scalar, image, dL/dh)."""
     h = (d + base).contiguous()
     n = normals(h)
-    Ep, xp, Mp, cp, radius = simulate(h, scene, emitter, n)
+    Ep, xp, Mp, cp, radius, hits = simulate(h, scene, emitter, n, keep_hits=True)
     img, bins = _forward(Ep, xp, Mp, cp, radius, size, cap, mode, None, keep_bins=True)
     resid, loss = residual(img, target)
     gE, gx, gM = _backward(resid, Ep, xp, Mp, cp, radius, cap, mode, None, bins=bins)
-    gh = adjoint(h, scene, emitter, gE, gx, gM, n)
+    gh = adjoint(h, scene, emitter, gE, gx, gM, n, hits=hits)
     landweber_update(d, gh, mask, params)
     return loss, img, gh

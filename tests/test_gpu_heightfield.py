@@ -239,3 +239,52 @@ def test_point_light_emitter_and_adjoint(light):
     err = float((gh - g).abs().max() / g.abs().max())
     print(f"point light {light}: dL/dh max err / max {err:.3e}")
     assert err <= 1e-3
+
+
+@pytest.mark.parametrize("case", ["parallel", "point", "grazing", "steep"])
+def test_adjoint_replaying_recorded_hits_matches_tracing(case):
+    # hf_simulate_hits_f32 records each photon's first hits; hf_adjoint_hits_f32
+    # replays them instead of tracing: the same operations on the same values,
+    # so the splat inputs equal the tracing pair's bit for bit and every
+    # photon's vertex-gradient terms too; dL/dh differs only by the order of
+    # the float atomic sums, which varies from run to run in either adjoint
+    # (paths with more bounces than recorded slots are traced again)
+    from shapefromcaustics_amd.PhotonDifferentialSplatting import _backward, _forward
+    R, n_side, sz, size, cap = 24, 48, -1.5, (1, 64, 64), 8
+    h = _h(R, 4, n_bumps=8, amp=(0.0, 0.12), sigma=(0.1, 0.4)).float().cuda()
+    sc = HF.Scene(R, sz)
+    if case == "steep":  # steep bumps: paths of 4 hits and paths longer than the record (traced again)
+        n_side = 64
+        h = _h(R, 4, n_bumps=12, amp=(0.4, 0.9), sigma=(0.04, 0.1)).float().cuda()
+    if case in ("parallel", "steep"):
+        em = HF.ParallelEmitter(n_side, 1.0)
+    elif case == "point":
+        em = HF.PointLight(n_side, (0.1, -0.2, 3.0), 0.2, flux=1e-3, smoothing=1.0, seed=7)
+    else:  # low light far to the side: grazing rays, wall hits and internal reflections
+        em = HF.PointLight(n_side, (-2.5, 0.3, 0.6), 0.1, flux=1e-3, smoothing=1.0, seed=3)
+    n = HF.normals(h)
+    a = HF.simulate(h, sc, em, n)
+    b = HF.simulate(h, sc, em, n, keep_hits=True)
+    for x, y in zip(a, b[:5]):
+        assert torch.equal(x, y)
+    hits = b[5]
+    N = a[0].numel()
+    counts = hits[-N:].cpu()
+    alive = (a[3][0] >= 0).cpu()
+    assert torch.equal(counts > 0, alive)
+    print(f"{case}: {int(alive.sum())} photons reach the sensor; hits recorded per photon:",
+          {int(k): int((counts == k).sum()) for k in torch.unique(counts[alive])})
+    if case == "steep":
+        assert (counts == 4).any() and (counts == 255).any()
+    img, bins = _forward(*a, size, cap, "auto", None, keep_bins=True)
+    g = torch.from_numpy(np.random.default_rng(1).standard_normal(size).astype(np.float32)).cuda()
+    gE, gx, gM = _backward(g, *a, cap, "auto", None, bins=bins)
+    g0 = HF.adjoint(h, sc, em, gE, gx, gM, n)
+    g0b = HF.adjoint(h, sc, em, gE, gx, gM, n)
+    g1 = HF.adjoint(h, sc, em, gE, gx, gM, n, hits=hits)
+    scale = float(g0.abs().max())
+    assert scale > 0
+    noise = float((g0 - g0b).abs().max()) / scale
+    diff = float((g0 - g1).abs().max()) / scale
+    print(f"{case}: run-to-run {noise:.2e}, replay vs trace {diff:.2e} (of max |dL/dh|)")
+    assert diff <= 1e-5
