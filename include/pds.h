@@ -50,7 +50,7 @@ typedef enum pds_status {
 } pds_status;
 
 typedef enum pds_mode {
-    PDS_MODE_AUTO = 0,          /* fastest exact mode (binned forward, cooperative adjoint) */
+    PDS_MODE_AUTO = 0,          /* fastest mode: tile-binned when the problem can be binned, else photon order */
     PDS_MODE_PLAIN = 1,         /* one thread per photon, global atomics: the reference's schedule */
     PDS_MODE_BINNED = 2,        /* tile-binned LDS forward, atomic flush (run-to-run order may vary) */
     PDS_MODE_DETERMINISTIC = 3  /* tile-ordered forward, bitwise reproducible run to run */
