@@ -33,14 +33,16 @@ int fail(int code, const char* msg) {
 inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // workspace: [0] hmax (float) ; then grad_n [R,R,3], grad_z [R,R], grad_g [R,R,2]
+constexpr int kHmaxBlocks = 128;
+
 struct WsLayout {
     size_t hmax, grad_n, grad_z, grad_g, end;
 };
 WsLayout ws_layout(int32_t R) {
     WsLayout L;
     size_t o = 0;
-    L.hmax = o;
-    o = align256(o + 256);
+    L.hmax = o;  // [0]: max h; [64, 64 + kHmaxBlocks): per-block maxima
+    o = align256(o + sizeof(float) * (64 + kHmaxBlocks));
     const size_t n = (size_t)R * R;
     L.grad_n = o;
     o = align256(o + sizeof(float) * 3 * n);
@@ -80,6 +82,21 @@ __global__ __launch_bounds__(256) void normals_kernel(const float* __restrict__ 
 }
 
 // max(h) into ws[0], one workgroup
+// max h in two stages: per-block maxima over a grid of kHmaxBlocks
+// workgroups (one workgroup alone walks 2^16 heights latency-bound), then
+// hmax_kernel over those maxima
+__global__ __launch_bounds__(256) void hmax_part_kernel(const float* __restrict__ h, int64_t n,
+                                                        float* __restrict__ part) {
+    __shared__ float wpart[4];
+    float m = -INFINITY;
+    for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (int64_t)gridDim.x * 256)
+        m = fmaxf(m, h[k]);
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) wpart[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = fmaxf(fmaxf(wpart[0], wpart[1]), fmaxf(wpart[2], wpart[3]));
+}
+
 __global__ __launch_bounds__(1024) void hmax_kernel(const float* __restrict__ h, int64_t n, float* __restrict__ out) {
     __shared__ float part[16];
     float m = -INFINITY;
@@ -804,6 +821,12 @@ __global__ __launch_bounds__(256) void vcons_kernel(const float* __restrict__ x,
 
 unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
 
+// hmax[0] = max of the n heights (hmax: the workspace's hmax region)
+void launch_hmax(const float* h, int64_t n, float* hmax, hipStream_t s) {
+    hipLaunchKernelGGL(hmax_part_kernel, dim3(kHmaxBlocks), dim3(256), 0, s, h, n, hmax + 64);
+    hipLaunchKernelGGL(hmax_kernel, dim3(1), dim3(1024), 0, s, (const float*)(hmax + 64), (int64_t)kHmaxBlocks, hmax);
+}
+
 EmitArgs emit_args(const hf_scene* sc, const hf_emitter* em) {
     EmitArgs E;
     E.n_side = em->n_side;
@@ -885,7 +908,7 @@ int simulate_impl(const float* h, const float* normals, const hf_scene* sc, cons
     float4* hrec = want_hits ? (float4*)((char*)hits + HL.recs) : nullptr;
     uint8_t* hcnt = want_hits ? (uint8_t*)hits + HL.count : nullptr;
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(hmax_kernel, dim3(1), dim3(1024), 0, s, h, (int64_t)sc->R * sc->R, hmax);
+    launch_hmax(h, (int64_t)sc->R * sc->R, hmax, s);
     hipLaunchKernelGGL(simulate_kernel, dim3(blocks_for(N)), dim3(256), 0, s, scene_args(h, normals, sc, hmax),
                        (const float*)hmax, emit_args(sc, em), O, hrec, hcnt);
     return hip_status("hf_simulate");
@@ -918,7 +941,7 @@ int adjoint_impl(const float* h, const float* normals, const hf_scene* sc, const
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(zero_kernel, dim3(blocks_for(3 * RR)), dim3(256), 0, s, gn, 3 * RR);
     hipLaunchKernelGGL(zero_kernel, dim3(blocks_for(RR)), dim3(256), 0, s, gz, RR);
-    hipLaunchKernelGGL(hmax_kernel, dim3(1), dim3(1024), 0, s, h, RR, hmax);
+    launch_hmax(h, RR, hmax, s);
     const int tiles_x = (em->n_side + 15) / 16;
     hipLaunchKernelGGL(adjoint_kernel, dim3((unsigned)(tiles_x * tiles_x), (unsigned)sc->channels), dim3(256), 0, s,
                        scene_args(h, normals, sc, hmax), (const float*)hmax, emit_args(sc, em), N, tiles_x, grad_Ep,
@@ -963,7 +986,7 @@ int hf_trace_f32(const float* h, const float* normals, const hf_scene* sc, const
     if (!ws || ws_bytes < L.end) return fail(PDS_ERR_WORKSPACE, "workspace smaller than hf_workspace_bytes(R)");
     float* hmax = (float*)((char*)ws + L.hmax);
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(hmax_kernel, dim3(1), dim3(1024), 0, s, h, (int64_t)sc->R * sc->R, hmax);
+    launch_hmax(h, (int64_t)sc->R * sc->R, hmax, s);
     hipLaunchKernelGGL(trace_kernel, dim3(blocks_for(N)), dim3(256), 0, s, scene_args(h, normals, sc, hmax),
                        (const float*)hmax, origins, dirs, N, t, uv, object, primitive);
     return hip_status("hf_trace");
