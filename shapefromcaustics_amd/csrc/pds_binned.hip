@@ -842,6 +842,8 @@ __device__ __forceinline__ int next_chunk(int* counter, int nchunks) {
     return c < nchunks ? nchunks - 1 - c : -1;
 }
 
+constexpr int kXcds = 8;  // MI355X: 8 accelerator complex dies, each with its own L2
+
 // Segment geometry: channel and tile of the segment's bin.
 struct SegTile {
     int c, X0, Y0, X1, Y1;
@@ -1161,10 +1163,20 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_bwd_kernel(Geom g, Imag
 __global__ __launch_bounds__(256) void bin_finalize_kernel(const float* __restrict__ grad, PhotonArgs P, ImageArgs I,
                                                            const Ctrl* ctrl, const uint8_t* __restrict__ info,
                                                            const uint32_t* __restrict__ slots,
-                                                           const float4* __restrict__ partials, GradArgs G) {
+                                                           const float4* __restrict__ partials, GradArgs G,
+                                                           int64_t per_block) {
+    // XCD-aware photon order.  Workgroups are dispatched round-robin over the
+    // 8 XCDs (workgroup b on XCD b % 8); the 256-photon chunks of one binning
+    // block go to workgroups of one XCD, at consecutive positions in that
+    // XCD's sequence.  A binning block's records lie in short runs (one per
+    // bin), so its photons' 32-byte partials share cache lines, which are
+    // then fetched once into that XCD's L2 instead of once per XCD.
     const int64_t N = P.N;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
+    const int64_t cpb = (per_block + 255) / 256;  // chunks per binning block
+    const int64_t x = blockIdx.x % kXcds, k = blockIdx.x / kXcds;
+    const int64_t blk = kXcds * (k / cpb) + x, off = (k % cpb) * 256 + threadIdx.x;
+    const int64_t i = blk * per_block + off;
+    if (off >= per_block || i >= N) return;
     // on overflow the scatter wrote no records, slots or classes
     const uint32_t cls = ctrl->overflow ? (uint32_t)kSpecial : (uint32_t)info[i];
     if (cls == kSpecial) {
@@ -1392,9 +1404,12 @@ void launch_backward_tiles(const float* grad, const PhotonArgs& P, const ImageAr
         hipLaunchKernelGGL(bin_tile_bwd_kernel<false>, dim3(nsegs), dim3(kTileThreads), tile_lds, s, g, I,
                            (const int4*)B.segs, (const uint4*)B.recs, (const Ctrl*)B.ctrl, grad, B.partials,
                            (const uint32_t*)nullptr);
-    const int64_t blocks = (P.N + 255) / 256;  // P.N > 0 here
+    // every (binning block, 256-photon chunk), binning blocks rounded up to a multiple of kXcds
+    const int64_t cpb = (g.per_block + 255) / 256;
+    const int64_t blocks = (((int64_t)g.G + kXcds - 1) / kXcds) * kXcds * cpb;
     hipLaunchKernelGGL(bin_finalize_kernel, dim3((unsigned)blocks), dim3(256), 0, s, grad, P, I, (const Ctrl*)B.ctrl,
-                       (const uint8_t*)B.info, (const uint32_t*)B.slots, (const float4*)B.partials, G);
+                       (const uint8_t*)B.info, (const uint32_t*)B.slots, (const float4*)B.partials, G,
+                       (int64_t)g.per_block);
 }
 
 // Photon-order forward for problems the binned path cannot take.
