@@ -197,76 +197,127 @@ __device__ __forceinline__ void mt_test(V3 o, V3 d, V3 v0, V3 v1, V3 v2, int obj
     if (best.obj < 0 || t < best.t) best = Hit{t, u, v, obj, prim};
 }
 
-__device__ __forceinline__ void test_prim(const SceneArgs& S, V3 o, V3 d, int obj, int prim, Hit& best) {
-    const TriVerts T = tri_verts(S, obj, prim);
-    mt_test(o, d, T.v[0], T.v[1], T.v[2], obj, prim, best);
-}
-
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-// Closest hit of the ray (o, d) with the scene.
+__device__ __forceinline__ V3 pick4(const V3& q0, const V3& q1, const V3& q2, const V3& q3, int k) {
+    const V3 lo = (k & 1) ? q1 : q0, hi = (k & 1) ? q3 : q2;
+    return (k & 2) ? hi : lo;
+}
+
+// The two triangles of a quad of object 0, primitives prim0 and prim0 + 1:
+// corners q0..q3, and per triangle three 2-bit corner indices (triangle 0 in
+// bits 0-5, triangle 1 in bits 6-11, vertex k of a triangle in bits 2k, 2k+1).
+// A compile-time code folds the corner selection away.
+__device__ __forceinline__ void quad_test(V3 o, V3 d, const V3& q0, const V3& q1, const V3& q2, const V3& q3,
+                                          int code, int prim0, Hit& best) {
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+        const int k = code >> (6 * half);
+        mt_test(o, d, pick4(q0, q1, q2, q3, k & 3), pick4(q0, q1, q2, q3, (k >> 2) & 3),
+                pick4(q0, q1, q2, q3, (k >> 4) & 3), 0, prim0 + half, best);
+    }
+}
+
+constexpr int tri_code(int a0, int a1, int a2, int b0, int b1, int b2) {
+    return a0 | a1 << 2 | a2 << 4 | b0 << 6 | b1 << 8 | b2 << 10;
+}
+// Corner order of a grid quad: q0 = (r, c), q1 = (r, c + 1), q2 = (r + 1, c),
+// q3 = (r + 1, c + 1).  Triangles as tri_verts lists them
+// (model/renderable_object.py:52-78):
+//   top    (r,c) (r,c+1) (r+1,c)      and (r,c+1) (r+1,c+1) (r+1,c)
+//   bottom (r,c) (r+1,c) (r,c+1)      and (r,c+1) (r+1,c) (r+1,c+1)
+constexpr int kTopCode = tri_code(0, 1, 2, 1, 3, 2);
+constexpr int kBottomCode = tri_code(0, 2, 1, 1, 2, 3);
+// Corner order of a wall segment between grid vertices A and B (B one step
+// further along the wall): q0 = A at z = 0, q1 = A on the top surface,
+// q2 = B at z = 0, q3 = B on the top surface.  Per wall, in trace order
+// (left x = -1, right x = +1, front y = -1, back y = +1):
+constexpr int kWallCode[4] = {tri_code(0, 1, 2, 1, 3, 2), tri_code(0, 2, 1, 1, 2, 3), tri_code(0, 2, 1, 2, 3, 1),
+                              tri_code(0, 1, 2, 2, 1, 3)};
+
+__device__ __forceinline__ bool same_hit(const Hit& a, const Hit& b) {
+    return a.prim == b.prim && a.obj == b.obj && a.t == b.t;
+}
+
+// Closest hit of the ray (o, d) with the scene.  The triangles are tested in
+// a fixed order (ground, bottom, walls, top surface); a later hit replaces
+// the best one only when strictly closer.  Hit values come from mt_test
+// alone; the plane parameters, clipping and cell walk below only choose which
+// triangles to test, so they multiply by one reciprocal per axis instead of
+// dividing.
 __device__ inline Hit trace(const SceneArgs& S, V3 o, V3 d) {
     Hit best = miss_hit();
     const int R = S.R, e = R - 1;
     const PrimBase B = prim_base(R);
-    // ground plane (2 triangles)
-    mt_test(o, d, v3(-1.f, -1.f, S.sensor_z), v3(1.f, -1.f, S.sensor_z), v3(-1.f, 1.f, S.sensor_z), 1, 0, best);
-    mt_test(o, d, v3(-1.f, 1.f, S.sensor_z), v3(1.f, -1.f, S.sensor_z), v3(1.f, 1.f, S.sensor_z), 1, 1, best);
+    const float rdx = 1.0f / d.x, rdy = 1.0f / d.y, rdz = 1.0f / d.z;  // inf on a zero component: never used then
+    // ground plane (2 triangles), skipped when it lies behind the ray (as for
+    // the bottom plane below: both triangles would fail t >= kTMin)
+    if (d.z != 0.0f && (S.sensor_z - o.z) * rdz >= 0.5f * kTMin) {
+        mt_test(o, d, v3(-1.f, -1.f, S.sensor_z), v3(1.f, -1.f, S.sensor_z), v3(-1.f, 1.f, S.sensor_z), 1, 0, best);
+        mt_test(o, d, v3(-1.f, 1.f, S.sensor_z), v3(1.f, -1.f, S.sensor_z), v3(1.f, 1.f, S.sensor_z), 1, 1, best);
+    }
     // bottom plane z = 0: the two triangles of the cell under the plane hit
-    // (its 3x3 neighbourhood only if rounding put the hit across a cell edge)
+    // (its 3x3 neighbourhood only if rounding put the hit across a cell edge).
+    // Every bottom triangle lies in that plane, so its intersection parameter
+    // is tp up to rounding (~1e-7 here): with tp < kTMin / 2 none of them can
+    // pass the t >= kTMin test -- the case of a ray leaving the bottom face,
+    // whose own cell and all 8 neighbours would otherwise be tested in vain.
     if (d.z != 0.0f) {
-        const float tp = -o.z / d.z;
-        if (tp >= 0.0f) {
+        const float tp = -o.z * rdz;
+        if (tp >= 0.5f * kTMin) {
             const float x = fmaf(tp, d.x, o.x), y = fmaf(tp, d.y, o.y);
             const float eps = 1e-5f;
             if (x >= -1.0f - eps && x <= 1.0f + eps && y >= -1.0f - eps && y <= 1.0f + eps) {
                 const int c = clampi((int)floorf((x + 1.0f) * S.inv_delta), 0, e - 1);
                 const int r = clampi((int)floorf((y + 1.0f) * S.inv_delta), 0, e - 1);
                 const Hit keep = best;
-                const int base = B.bottom + 2 * (r * e + c);
-                test_prim(S, o, d, 0, base, best);
-                test_prim(S, o, d, 0, base + 1, best);
-                const bool found = best.prim != keep.prim || best.obj != keep.obj || best.t != keep.t;
-                if (!found)
-                    for (int dr = -1; dr <= 1; ++dr)
-                        for (int dc = -1; dc <= 1; ++dc) {
-                            const int rr = r + dr, cc = c + dc;
-                            if ((dr == 0 && dc == 0) || rr < 0 || rr >= e || cc < 0 || cc >= e) continue;
-                            const int nb = B.bottom + 2 * (rr * e + cc);
-                            test_prim(S, o, d, 0, nb, best);
-                            test_prim(S, o, d, 0, nb + 1, best);
-                        }
+#pragma unroll 1
+                for (int j = 0; j < 9; ++j) {
+                    // j = 0: the cell itself; 1..8: its neighbours, row-major, if it held no hit
+                    if (j == 1 && !same_hit(best, keep)) break;
+                    int rr = r, cc = c;
+                    if (j > 0) {
+                        const int q = j - 1 + (j > 4 ? 1 : 0);
+                        rr = r + q / 3 - 1;
+                        cc = c + q % 3 - 1;
+                        if (rr < 0 || rr >= e || cc < 0 || cc >= e) continue;
+                    }
+                    const float x0 = coord(S, cc), x1 = coord(S, cc + 1), y0 = coord(S, rr), y1 = coord(S, rr + 1);
+                    quad_test(o, d, v3(x0, y0, 0.f), v3(x1, y0, 0.f), v3(x0, y1, 0.f), v3(x1, y1, 0.f), kBottomCode,
+                              B.bottom + 2 * (rr * e + cc), best);
+                }
             }
         }
     }
-    // side walls x = +-1, y = +-1: the two triangles of the segment at the plane
-    // hit, when that hit lies within the wall's extent (neighbours on a miss)
-    auto wall = [&](float tp, float lat, float z, int base) {
-        if (!(tp >= 0.0f) || !(z >= -1e-5f) || !(z <= S.hmax + 1e-5f) || !(fabsf(lat) <= 1.0f + 1e-5f)) return;
+    // side walls x = -1, x = +1, y = -1, y = +1: the two triangles of the
+    // segment at the plane hit, when that hit lies within the wall's extent
+    // (the neighbouring segments if they miss)
+#pragma unroll 1
+    for (int w = 0; w < 4; ++w) {
+        const bool xw = w < 2;
+        const float dd = xw ? d.x : d.y;
+        if (dd == 0.0f) continue;
+        const float P = (w & 1) ? 1.0f : -1.0f;
+        const float tp = (P - (xw ? o.x : o.y)) * (xw ? rdx : rdy);
+        const float lat = xw ? fmaf(tp, d.y, o.y) : fmaf(tp, d.x, o.x);
+        const float z = fmaf(tp, d.z, o.z);
+        if (!(tp >= 0.0f) || !(z >= -1e-5f) || !(z <= S.hmax + 1e-5f) || !(fabsf(lat) <= 1.0f + 1e-5f)) continue;
         const int k = clampi((int)floorf((lat + 1.0f) * S.inv_delta), 0, e - 1);
+        const int base = w == 0 ? B.left : (w == 1 ? B.right : (w == 2 ? B.front : B.back));
+        const int code = w == 0 ? kWallCode[0] : (w == 1 ? kWallCode[1] : (w == 2 ? kWallCode[2] : kWallCode[3]));
+        const int fixed = (w & 1) ? e : 0;  // the wall's grid column (x walls) or row (y walls)
         const Hit keep = best;
-        test_prim(S, o, d, 0, base + 2 * k, best);
-        test_prim(S, o, d, 0, base + 2 * k + 1, best);
-        if (best.prim == keep.prim && best.obj == keep.obj && best.t == keep.t)
-            for (int dk = -1; dk <= 1; dk += 2) {
-                const int kk = k + dk;
-                if (kk < 0 || kk >= e) continue;
-                test_prim(S, o, d, 0, base + 2 * kk, best);
-                test_prim(S, o, d, 0, base + 2 * kk + 1, best);
-            }
-    };
-    if (d.x != 0.0f) {
-        for (int side = 0; side < 2; ++side) {
-            const float X = side ? 1.0f : -1.0f;
-            const float tp = (X - o.x) / d.x;
-            wall(tp, fmaf(tp, d.y, o.y), fmaf(tp, d.z, o.z), side ? B.right : B.left);
-        }
-    }
-    if (d.y != 0.0f) {
-        for (int side = 0; side < 2; ++side) {
-            const float Y = side ? 1.0f : -1.0f;
-            const float tp = (Y - o.y) / d.y;
-            wall(tp, fmaf(tp, d.x, o.x), fmaf(tp, d.z, o.z), side ? B.back : B.front);
+#pragma unroll 1
+        for (int j = 0; j < 3; ++j) {
+            // j = 0: the segment itself; then k - 1 and k + 1 if it held no hit
+            if (j == 1 && !same_hit(best, keep)) break;
+            const int kk = k + (j == 0 ? 0 : (j == 1 ? -1 : 1));
+            if (kk < 0 || kk >= e) continue;
+            const int ra = xw ? kk : fixed, ca = xw ? fixed : kk;
+            const int rb = xw ? kk + 1 : fixed, cb = xw ? fixed : kk + 1;
+            const float xa = coord(S, ca), ya = coord(S, ra), xb = coord(S, cb), yb = coord(S, rb);
+            quad_test(o, d, v3(xa, ya, 0.f), v3(xa, ya, height(S, ra, ca)), v3(xb, yb, 0.f),
+                      v3(xb, yb, height(S, rb, cb)), code, base + 2 * kk, best);
         }
     }
     // top surface: walk the cells under the ray's xy path while its z lies in
@@ -275,12 +326,12 @@ __device__ inline Hit trace(const SceneArgs& S, V3 o, V3 d) {
     {
         float t0 = 0.0f, t1 = INFINITY;
         // clip to the square [-1, 1]^2
-        auto clip = [&](float oo, float dd) {
+        auto clip = [&](float oo, float dd, float rd) {
             if (dd == 0.0f) {
                 if (oo < -1.0f || oo > 1.0f) t1 = -1.0f;
                 return;
             }
-            float a = (-1.0f - oo) / dd, b = (1.0f - oo) / dd;
+            float a = (-1.0f - oo) * rd, b = (1.0f - oo) * rd;
             if (a > b) {
                 const float tmp = a;
                 a = b;
@@ -289,13 +340,13 @@ __device__ inline Hit trace(const SceneArgs& S, V3 o, V3 d) {
             t0 = fmaxf(t0, a);
             t1 = fminf(t1, b);
         };
-        clip(o.x, d.x);
-        clip(o.y, d.y);
+        clip(o.x, d.x, rdx);
+        clip(o.y, d.y, rdy);
         // clip to the slab 0 <= z <= hmax
         if (d.z == 0.0f) {
             if (o.z < 0.0f || o.z > S.hmax) t1 = -1.0f;
         } else {
-            float a = (0.0f - o.z) / d.z, b = (S.hmax - o.z) / d.z;
+            float a = (0.0f - o.z) * rdz, b = (S.hmax - o.z) * rdz;
             if (a > b) {
                 const float tmp = a;
                 a = b;
@@ -312,16 +363,17 @@ __device__ inline Hit trace(const SceneArgs& S, V3 o, V3 d) {
             int r = clampi((int)floorf((y + 1.0f) * S.inv_delta), 0, e - 1);
             const int sc = d.x > 0.0f ? 1 : (d.x < 0.0f ? -1 : 0);
             const int sr = d.y > 0.0f ? 1 : (d.y < 0.0f ? -1 : 0);
-            const float tdx = sc ? S.delta / fabsf(d.x) : INFINITY;
-            const float tdy = sr ? S.delta / fabsf(d.y) : INFINITY;
-            float tnx = sc ? ((coord(S, c + (sc > 0 ? 1 : 0)) - o.x) / d.x) : INFINITY;
-            float tny = sr ? ((coord(S, r + (sr > 0 ? 1 : 0)) - o.y) / d.y) : INFINITY;
+            const float tdx = sc ? S.delta * fabsf(rdx) : INFINITY;
+            const float tdy = sr ? S.delta * fabsf(rdy) : INFINITY;
+            float tnx = sc ? ((coord(S, c + (sc > 0 ? 1 : 0)) - o.x) * rdx) : INFINITY;
+            float tny = sr ? ((coord(S, r + (sr > 0 ? 1 : 0)) - o.y) * rdy) : INFINITY;
             Hit cell_best = miss_hit();
+#pragma unroll 1
             for (int guard = 0; guard < 4 * R + 8; ++guard) {
-                // this cell and, against float edge cases, its previous neighbours
-                const int base = B.top + 2 * (r * e + c);
-                test_prim(S, o, d, 0, base, cell_best);
-                test_prim(S, o, d, 0, base + 1, cell_best);
+                const float x0 = coord(S, c), x1 = coord(S, c + 1), y0 = coord(S, r), y1 = coord(S, r + 1);
+                quad_test(o, d, v3(x0, y0, height(S, r, c)), v3(x1, y0, height(S, r, c + 1)),
+                          v3(x0, y1, height(S, r + 1, c)), v3(x1, y1, height(S, r + 1, c + 1)), kTopCode,
+                          B.top + 2 * (r * e + c), cell_best);
                 const float texit = fminf(tnx, tny);
                 if (cell_best.obj >= 0 && cell_best.t <= texit * (1.0f + 1e-6f) + 1e-6f) break;
                 if (texit > t1) break;
