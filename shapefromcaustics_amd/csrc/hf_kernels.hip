@@ -438,12 +438,15 @@ __device__ __forceinline__ void transfer1_adj(V3 ng, V3 d, float t, V3 Du, V3 Dt
 
 // Per-vertex gradient accumulator of one workgroup: a PxP patch of top
 // vertices in LDS (the rays of a 16x16 ray tile land on a few cells), global
-// atomics for vertices outside the patch.
+// atomics for vertices outside the patch.  The patch sums in double: gfx950's
+// LDS float add serialises a wavefront's lanes (~190 cycles per instruction
+// per CU) while its double add does not (~9), and the sum rounds to float
+// once, at the flush.
 constexpr int kPatch = 8;
 
 struct PatchAcc {
-    float* n;  // [kPatch][kPatch][3] in LDS
-    float* z;  // [kPatch][kPatch] in LDS
+    double* n;  // [kPatch][kPatch][3] in LDS
+    double* z;  // [kPatch][kPatch] in LDS
     int r0, c0, R;
     float* grad_n;
     float* grad_z;
@@ -455,9 +458,9 @@ struct PatchAcc {
     __device__ __forceinline__ void add_n(int r, int c, V3 v) const {
         int k;
         if (inside(r, c, k)) {
-            atomicAdd(n + 3 * k, v.x);
-            atomicAdd(n + 3 * k + 1, v.y);
-            atomicAdd(n + 3 * k + 2, v.z);
+            atomicAdd(n + 3 * k, (double)v.x);
+            atomicAdd(n + 3 * k + 1, (double)v.y);
+            atomicAdd(n + 3 * k + 2, (double)v.z);
         } else {
             atomic_add3(grad_n + 3 * (r * R + c), v);
         }
@@ -465,7 +468,7 @@ struct PatchAcc {
     __device__ __forceinline__ void add_z(int r, int c, float v) const {
         int k;
         if (inside(r, c, k))
-            atomicAdd(z + k, v);
+            atomicAdd(z + k, (double)v);
         else
             unsafeAtomicAdd(grad_z + r * R + c, v);
     }
@@ -637,14 +640,14 @@ __global__ __launch_bounds__(256) void adjoint_kernel(SceneArgs S, const float* 
                                                       float* __restrict__ grad_n, float* __restrict__ grad_z,
                                                       const float4* __restrict__ hits,
                                                       const uint8_t* __restrict__ nhit) {
-    __shared__ float pn[kPatch * kPatch * 3];
-    __shared__ float pz[kPatch * kPatch];
+    __shared__ double pn[kPatch * kPatch * 3];
+    __shared__ double pz[kPatch * kPatch];
     S.hmax = hmax[0];
     const int ch = blockIdx.y;
     const int tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
     const int col = tx * 16 + (threadIdx.x & 15), row = ty * 16 + (threadIdx.x >> 4);
-    for (int k = threadIdx.x; k < kPatch * kPatch * 3; k += 256) pn[k] = 0.0f;
-    for (int k = threadIdx.x; k < kPatch * kPatch; k += 256) pz[k] = 0.0f;
+    for (int k = threadIdx.x; k < kPatch * kPatch * 3; k += 256) pn[k] = 0.0;
+    for (int k = threadIdx.x; k < kPatch * kPatch; k += 256) pz[k] = 0.0;
     PatchAcc A;
     A.n = pn;
     A.z = pz;
@@ -665,9 +668,9 @@ __global__ __launch_bounds__(256) void adjoint_kernel(SceneArgs S, const float* 
     for (int k = threadIdx.x; k < kPatch * kPatch; k += 256) {
         const int r = A.r0 + k / kPatch, c = A.c0 + k % kPatch;
         if (r < 0 || r >= S.R || c < 0 || c >= S.R) continue;  // never accumulated: outside the grid
-        const float vz = pz[k];
+        const float vz = (float)pz[k];
         if (vz != 0.0f) unsafeAtomicAdd(grad_z + r * S.R + c, vz);
-        const V3 vn = v3(pn[3 * k], pn[3 * k + 1], pn[3 * k + 2]);
+        const V3 vn = v3((float)pn[3 * k], (float)pn[3 * k + 1], (float)pn[3 * k + 2]);
         if (vn.x != 0.0f || vn.y != 0.0f || vn.z != 0.0f) atomic_add3(grad_n + 3 * (r * S.R + c), vn);
     }
 }
