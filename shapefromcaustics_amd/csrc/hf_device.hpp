@@ -23,6 +23,7 @@
 namespace hf {
 
 constexpr float kTMin = 1e-3f;
+constexpr float kCullZ = 1e-3f;  // height margin of the cell cull (float rounding of z is ~1e-7 here)
 constexpr float kMissT = -2.0f;
 constexpr int kMaxBounces = 8;
 
@@ -368,15 +369,33 @@ __device__ inline Hit trace(const SceneArgs& S, V3 o, V3 d) {
             float tnx = sc ? ((coord(S, c + (sc > 0 ? 1 : 0)) - o.x) * rdx) : INFINITY;
             float tny = sr ? ((coord(S, r + (sr > 0 ? 1 : 0)) - o.y) * rdy) : INFINITY;
             Hit cell_best = miss_hit();
+            float tcur = t0;  // where the ray enters the current cell (or starts)
 #pragma unroll 1
             for (int guard = 0; guard < 4 * R + 8; ++guard) {
-                const float x0 = coord(S, c), x1 = coord(S, c + 1), y0 = coord(S, r), y1 = coord(S, r + 1);
-                quad_test(o, d, v3(x0, y0, height(S, r, c)), v3(x1, y0, height(S, r, c + 1)),
-                          v3(x0, y1, height(S, r + 1, c)), v3(x1, y1, height(S, r + 1, c + 1)), kTopCode,
-                          B.top + 2 * (r * e + c), cell_best);
+                const float h00 = height(S, r, c), h01 = height(S, r, c + 1);
+                const float h10 = height(S, r + 1, c), h11 = height(S, r + 1, c + 1);
                 const float texit = fminf(tnx, tny);
+                // Cull: a hit on the cell's triangles lies over the cell, i.e.
+                // at t in [tcur, texit], and between the cell's lowest and
+                // highest vertex; if the ray's z over that interval stays
+                // clear of those heights (by far more than rounding) the
+                // triangles cannot be hit.  Rays inside the glass, below the
+                // surface, skip most cells this way.
+                bool test = true;
+                if (texit < INFINITY) {
+                    const float za = fmaf(tcur, d.z, o.z), zb = d.z != 0.0f ? fmaf(texit, d.z, o.z) : o.z;
+                    const float cmin = fminf(fminf(h00, h01), fminf(h10, h11));
+                    const float cmax = fmaxf(fmaxf(h00, h01), fmaxf(h10, h11));
+                    test = fmaxf(za, zb) >= cmin - kCullZ && fminf(za, zb) <= cmax + kCullZ;
+                }
+                if (test) {
+                    const float x0 = coord(S, c), x1 = coord(S, c + 1), y0 = coord(S, r), y1 = coord(S, r + 1);
+                    quad_test(o, d, v3(x0, y0, h00), v3(x1, y0, h01), v3(x0, y1, h10), v3(x1, y1, h11), kTopCode,
+                              B.top + 2 * (r * e + c), cell_best);
+                }
                 if (cell_best.obj >= 0 && cell_best.t <= texit * (1.0f + 1e-6f) + 1e-6f) break;
                 if (texit > t1) break;
+                tcur = texit;
                 if (tnx < tny) {
                     c += sc;
                     tnx += tdx;

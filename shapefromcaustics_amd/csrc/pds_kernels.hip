@@ -105,17 +105,19 @@ __global__ __launch_bounds__(256) void fwd_flat_kernel(PhotonArgs P, ImageArgs I
     }
 }
 
-// Backward: gather + per-photon LDS reduction, coalesced stores.
+// Backward: gather + per-photon LDS reduction, coalesced stores.  The LDS
+// sums are double: gfx950's LDS float add serialises a wavefront's lanes
+// (~190 cycles per instruction per CU), its double add does not (~9).
 template <int U>
 __global__ __launch_bounds__(256) void bwd_flat_kernel(const float* __restrict__ grad, PhotonArgs P, ImageArgs I,
                                                        GradArgs G) {
     __shared__ WaveRecs recs[4];
-    __shared__ float acc[4][7][64];
+    __shared__ double acc[4][7][64];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     WaveRecs& R = recs[w];
     R.mark[lane] = -1;
 #pragma unroll
-    for (int k = 0; k < 7; ++k) acc[w][k][lane] = 0.f;
+    for (int k = 0; k < 7; ++k) acc[w][k][lane] = 0.0;
     const int64_t nbatch = (P.N + 63) >> 6;
     for (int64_t b = (int64_t)blockIdx.x * 4 + w; b < nbatch; b += (int64_t)gridDim.x * 4) {
         const int64_t i = (b << 6) + lane;
@@ -138,14 +140,14 @@ __global__ __launch_bounds__(256) void bwd_flat_kernel(const float* __restrict__
                 }
             }
 #pragma unroll
-            for (int k = 0; k < 7; ++k) atomicAdd(&acc[w][k][it.rec], a[k]);
+            for (int k = 0; k < 7; ++k) atomicAdd(&acc[w][k][it.rec], (double)a[k]);
         });
         wave_lds_fence();
         float s[7];
 #pragma unroll
         for (int k = 0; k < 7; ++k) {
-            s[k] = acc[w][k][lane];
-            acc[w][k][lane] = 0.f;
+            s[k] = (float)acc[w][k][lane];
+            acc[w][k][lane] = 0.0;
         }
         if (i < P.N) store_grads(G, I, P.N, i, s);
         wave_lds_fence();

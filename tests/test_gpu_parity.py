@@ -243,6 +243,20 @@ def test_deterministic_outside_binned_range(size, cap):
     assert torch.equal(a, b)
 
 
+@pytest.mark.parametrize("mode", ["auto", "binned"])
+def test_cap_above_binned_range_uses_photon_order_kernels(mode):
+    # cap 70 > 63 on a 256^2 image: the window ellipse test stays inactive
+    # (2 * 70 / 256 < 1 / sqrt 2), so the flat photon-order kernels run
+    p = WL.plain(400, (1, 256, 256), 70, seed=9)
+    s = np.random.default_rng(9).uniform(20, 90, p.n)  # windows up to and past the cap
+    p = WL.from_pixel_space((p.xp[0] + 1) * 128, (p.xp[1] + 1) * 128, np.zeros(p.n), s, s * 0.6, p.E, p.cp[0],
+                            p.size, 70)
+    g = WL.upstream_grad(p.size, seed=9)
+    img, grads = _run(p, mode, g)
+    assert_image_close(img, O.forward_f64(*p.arrays(), p.size, p.cap), "cap 70 fwd")
+    assert_grads_close(grads, O.backward_f64(g, *p.arrays(), p.cap), "cap 70 bwd")
+
+
 def test_deterministic_mode_is_bitwise_reproducible():
     p = WL.caustic(1 << 20, (3, 512, 512), 12, seed=5)
     E, xp, M, cp, radius = _dev(p)
