@@ -88,12 +88,39 @@ struct TriVerts {
     int vr[3], vc[3];  // grid vertex of a top vertex, vr = -1 for constant vertices
 };
 
+constexpr int tri_code(int a0, int a1, int a2, int b0, int b1, int b2) {
+    return a0 | a1 << 2 | a2 << 4 | b0 << 6 | b1 << 8 | b2 << 10;
+}
+// Every primitive of object 0 is one of the two triangles of a quad.  Corner
+// order of a grid quad: q0 = (r, c), q1 = (r, c + 1), q2 = (r + 1, c),
+// q3 = (r + 1, c + 1); triangles (model/renderable_object.py:52-78, the two
+// halves of prim / 2 in grid_shuffle order):
+//   top    (r,c) (r,c+1) (r+1,c)      and (r,c+1) (r+1,c+1) (r+1,c)
+//   bottom (r,c) (r+1,c) (r,c+1)      and (r,c+1) (r+1,c) (r+1,c+1)
+// A code holds per triangle three 2-bit corner indices (triangle 0 in bits
+// 0-5, triangle 1 in bits 6-11, vertex k in bits 2k, 2k+1).
+constexpr int kTopCode = tri_code(0, 1, 2, 1, 3, 2);
+constexpr int kBottomCode = tri_code(0, 2, 1, 1, 2, 3);
+// Corner order of a wall segment between grid vertices A and B (B one step
+// further along the wall): q0 = A at z = 0, q1 = A on the top surface,
+// q2 = B at z = 0, q3 = B on the top surface.  Per wall, in the order
+// left x = -1, right x = +1, front y = -1, back y = +1:
+constexpr int kWallCode[4] = {tri_code(0, 1, 2, 1, 3, 2), tri_code(0, 2, 1, 1, 2, 3), tri_code(0, 2, 1, 2, 3, 1),
+                              tri_code(0, 1, 2, 2, 1, 3)};
+
+__device__ __forceinline__ int pick4i(int a, int b, int c, int d, int k) {
+    const int lo = (k & 1) ? b : a, hi = (k & 1) ? d : c;
+    return (k & 2) ? hi : lo;
+}
+
 // Vertices and vertex normals of (obj, prim) -- grid_shuffle order
-// (model/renderable_object.py:52-78).
+// (model/renderable_object.py:52-78).  The primitive is described by its
+// quad (four corner grid vertices, which of them lie on the top surface, the
+// face normal unless the top surface's vertex normals apply) and its corner
+// code; the three vertices are then picked with constant array indices, so
+// the result stays in registers.
 __device__ __forceinline__ TriVerts tri_verts(const SceneArgs& S, int obj, int prim) {
     TriVerts T;
-    const int R = S.R, e = R - 1;
-    for (int k = 0; k < 3; ++k) T.vr[k] = -1, T.vc[k] = 0;
     if (obj == 1) {  // ground plane
         const float z = S.sensor_z;
         if (prim == 0) {
@@ -101,74 +128,65 @@ __device__ __forceinline__ TriVerts tri_verts(const SceneArgs& S, int obj, int p
         } else {
             T.v[0] = v3(-1.f, 1.f, z), T.v[1] = v3(1.f, -1.f, z), T.v[2] = v3(1.f, 1.f, z);
         }
-        for (int k = 0; k < 3; ++k) T.n[k] = v3(0.f, 0.f, 1.f);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            T.n[k] = v3(0.f, 0.f, 1.f);
+            T.vr[k] = -1;
+            T.vc[k] = 0;
+        }
         return T;
     }
+    const int R = S.R, e = R - 1;
     const PrimBase B = prim_base(R);
-    auto top = [&](int k, int r, int c) {
-        T.v[k] = top_pt(S, r, c);
-        const float* nn = S.normals + 3 * (r * R + c);
-        T.n[k] = v3(nn[0], nn[1], nn[2]);
-        T.vr[k] = r;
-        T.vc[k] = c;
-    };
-    auto bot = [&](int k, int r, int c, V3 n) {
-        T.v[k] = bot_pt(S, r, c);
-        T.n[k] = n;
-    };
-    auto wall = [&](int k, int r, int c, V3 n) {  // top vertex of a wall: height from h, constant normal
-        T.v[k] = top_pt(S, r, c);
-        T.n[k] = n;
-        T.vr[k] = r;
-        T.vc[k] = c;
-    };
-    if (prim < B.bottom) {
-        const int cell = prim >> 1, r = cell / e, c = cell - r * e;
-        if ((prim & 1) == 0) {
-            top(0, r, c), top(1, r, c + 1), top(2, r + 1, c);
+    int r0, c0, r1, c1, r2, c2, r3, c3;  // corner grid vertices
+    int topmask, code;                   // corners on the top surface (bit k: corner k)
+    bool grid_normals = false;           // top surface: the vertices' own normals
+    V3 fn = v3(0.f, 0.f, 0.f);           // else the face normal
+    if (prim < B.front) {  // top or bottom: a grid cell
+        const bool top = prim < B.bottom;
+        const int cell = (top ? prim : prim - B.bottom) >> 1, r = cell / e, c = cell - r * e;
+        r0 = r, c0 = c, r1 = r, c1 = c + 1, r2 = r + 1, c2 = c, r3 = r + 1, c3 = c + 1;
+        topmask = top ? 15 : 0;
+        code = top ? kTopCode : kBottomCode;
+        grid_normals = top;
+        if (!top) fn = v3(0.f, 0.f, -1.f);
+    } else {  // a wall segment: corners A, A, B, B
+        int w, k;
+        if (prim < B.back) {
+            w = 2, k = (prim - B.front) >> 1;
+        } else if (prim < B.left) {
+            w = 3, k = (prim - B.back) >> 1;
+        } else if (prim < B.right) {
+            w = 0, k = (prim - B.left) >> 1;
         } else {
-            top(0, r, c + 1), top(1, r + 1, c + 1), top(2, r + 1, c);
+            w = 1, k = (prim - B.right) >> 1;
         }
-    } else if (prim < B.front) {
-        const V3 n = v3(0.f, 0.f, -1.f);
-        const int q = prim - B.bottom, cell = q >> 1, r = cell / e, c = cell - r * e;
-        if ((q & 1) == 0) {
-            bot(0, r, c, n), bot(1, r + 1, c, n), bot(2, r, c + 1, n);
+        const bool xw = w < 2;
+        const int fixed = (w & 1) ? e : 0;  // column of an x wall, row of a y wall
+        r0 = r1 = xw ? k : fixed;
+        c0 = c1 = xw ? fixed : k;
+        r2 = r3 = xw ? k + 1 : fixed;
+        c2 = c3 = xw ? fixed : k + 1;
+        topmask = 2 | 8;
+        code = w == 0 ? kWallCode[0] : (w == 1 ? kWallCode[1] : (w == 2 ? kWallCode[2] : kWallCode[3]));
+        const float sgn = (w & 1) ? 1.0f : -1.0f;
+        fn = xw ? v3(sgn, 0.f, 0.f) : v3(0.f, sgn, 0.f);
+    }
+    const int sh = 6 * (prim & 1);  // quad halves are even/odd primitives: every base is even
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int q = (code >> (sh + 2 * k)) & 3;
+        const int r = pick4i(r0, r1, r2, r3, q), c = pick4i(c0, c1, c2, c3, q);
+        const bool on_top = (topmask >> q) & 1;
+        T.v[k] = v3(coord(S, c), coord(S, r), on_top ? height(S, r, c) : 0.0f);
+        if (grid_normals) {
+            const float* nn = S.normals + 3 * (r * R + c);
+            T.n[k] = v3(nn[0], nn[1], nn[2]);
         } else {
-            bot(0, r, c + 1, n), bot(1, r + 1, c, n), bot(2, r + 1, c + 1, n);
+            T.n[k] = fn;
         }
-    } else if (prim < B.back) {  // front, y = -1: grid rows (bottom, top) of row 0
-        const V3 n = v3(0.f, -1.f, 0.f);
-        const int q = prim - B.front, c = q >> 1;
-        if ((q & 1) == 0) {
-            bot(0, 0, c, n), bot(1, 0, c + 1, n), wall(2, 0, c, n);
-        } else {
-            bot(0, 0, c + 1, n), wall(1, 0, c + 1, n), wall(2, 0, c, n);
-        }
-    } else if (prim < B.left) {  // back, y = +1, flipped
-        const V3 n = v3(0.f, 1.f, 0.f);
-        const int q = prim - B.back, c = q >> 1;
-        if ((q & 1) == 0) {
-            bot(0, e, c, n), wall(1, e, c, n), bot(2, e, c + 1, n);
-        } else {
-            bot(0, e, c + 1, n), wall(1, e, c, n), wall(2, e, c + 1, n);
-        }
-    } else if (prim < B.right) {  // left, x = -1: grid columns (bottom, top) of column 0
-        const V3 n = v3(-1.f, 0.f, 0.f);
-        const int q = prim - B.left, r = q >> 1;
-        if ((q & 1) == 0) {
-            bot(0, r, 0, n), wall(1, r, 0, n), bot(2, r + 1, 0, n);
-        } else {
-            wall(0, r, 0, n), wall(1, r + 1, 0, n), bot(2, r + 1, 0, n);
-        }
-    } else {  // right, x = +1, flipped
-        const V3 n = v3(1.f, 0.f, 0.f);
-        const int q = prim - B.right, r = q >> 1;
-        if ((q & 1) == 0) {
-            bot(0, r, e, n), bot(1, r + 1, e, n), wall(2, r, e, n);
-        } else {
-            wall(0, r, e, n), bot(1, r + 1, e, n), wall(2, r + 1, e, n);
-        }
+        T.vr[k] = on_top ? r : -1;
+        T.vc[k] = on_top ? c : 0;
     }
     return T;
 }
@@ -218,23 +236,6 @@ __device__ __forceinline__ void quad_test(V3 o, V3 d, const V3& q0, const V3& q1
                 pick4(q0, q1, q2, q3, (k >> 4) & 3), 0, prim0 + half, best);
     }
 }
-
-constexpr int tri_code(int a0, int a1, int a2, int b0, int b1, int b2) {
-    return a0 | a1 << 2 | a2 << 4 | b0 << 6 | b1 << 8 | b2 << 10;
-}
-// Corner order of a grid quad: q0 = (r, c), q1 = (r, c + 1), q2 = (r + 1, c),
-// q3 = (r + 1, c + 1).  Triangles as tri_verts lists them
-// (model/renderable_object.py:52-78):
-//   top    (r,c) (r,c+1) (r+1,c)      and (r,c+1) (r+1,c+1) (r+1,c)
-//   bottom (r,c) (r+1,c) (r,c+1)      and (r,c+1) (r+1,c) (r+1,c+1)
-constexpr int kTopCode = tri_code(0, 1, 2, 1, 3, 2);
-constexpr int kBottomCode = tri_code(0, 2, 1, 1, 2, 3);
-// Corner order of a wall segment between grid vertices A and B (B one step
-// further along the wall): q0 = A at z = 0, q1 = A on the top surface,
-// q2 = B at z = 0, q3 = B on the top surface.  Per wall, in trace order
-// (left x = -1, right x = +1, front y = -1, back y = +1):
-constexpr int kWallCode[4] = {tri_code(0, 1, 2, 1, 3, 2), tri_code(0, 2, 1, 1, 2, 3), tri_code(0, 2, 1, 2, 3, 1),
-                              tri_code(0, 1, 2, 2, 1, 3)};
 
 __device__ __forceinline__ bool same_hit(const Hit& a, const Hit& b) {
     return a.prim == b.prim && a.obj == b.obj && a.t == b.t;

@@ -165,6 +165,15 @@ struct PathState {
     V3 o, d, Du, Dv, Dt, Dp;
 };
 
+// E.ior[ch] by selects: a dynamic index into the kernel-argument array would
+// copy it to scratch memory first
+__device__ __forceinline__ float ior_of(const EmitArgs& E, int ch) {
+    float v = E.ior[0];
+#pragma unroll
+    for (int k = 1; k < HF_MAX_CHANNELS; ++k) v = ch == k ? E.ior[k] : v;
+    return v;
+}
+
 __device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
     // SplitMix64 (Steele, Lea and Flood, OOPSLA 2014)
     z += 0x9E3779B97F4A7C15ull;
@@ -342,7 +351,7 @@ __global__ __launch_bounds__(256) void simulate_kernel(SceneArgs S, const float*
     S.hmax = hmax[0];
     int ch;
     PathState s = emit(E, i, ch);
-    const float ei = E.ior[ch], eo = 1.0f;
+    const float ei = ior_of(E, ch), eo = 1.0f;
     int nb = 0;
     for (int bounce = 0; bounce < kMaxBounces; ++bounce) {
         const Hit hit = trace(S, s.o, s.d);
@@ -485,14 +494,17 @@ __device__ __forceinline__ void adjoint_photon(const SceneArgs& S, const EmitArg
     const bool replay = m != kHitsRetrace;
     int ch;
     const PathState s0 = emit(E, i, ch);
-    const float ei = E.ior[ch], eo = 1.0f;
+    const float ei = ior_of(E, ch), eo = 1.0f;
     auto hit_at = [&](int b) { return unpack_hit(hits[(int64_t)b * N + i]); };
     Tape tape[kMaxBounces];  // traced paths only
     int nb = 0;
-    PathState s = s0;
+    PathState s = s0, s1 = s0;  // s1: the state after the first bounce (replay)
     if (replay) {
         nb = m;  // the last recorded hit is the ground
-        for (int b = 0; b + 1 < m; ++b) advance(S, s, hit_at(b), eo, ei);
+        for (int b = 0; b + 1 < m; ++b) {
+            advance(S, s, hit_at(b), eo, ei);
+            if (b == 0) s1 = s;
+        }
     } else {
         bool reached = false;
         // forward, recording
@@ -519,8 +531,9 @@ __device__ __forceinline__ void adjoint_photon(const SceneArgs& S, const EmitArg
     auto bounce_at = [&](int k) -> Tape {
         if (!replay) return tape[k];
         if (k == nb - 1) return tape_of(s, hit_at(k));
-        PathState r = s0;
-        for (int b = 0; b < k; ++b) advance(S, r, hit_at(b), eo, ei);
+        if (k == 0) return tape_of(s0, hit_at(0));
+        PathState r = s1;
+        for (int b = 1; b < k; ++b) advance(S, r, hit_at(b), eo, ei);
         return tape_of(r, hit_at(k));
     };
     // upstream gradients of the splat inputs
