@@ -241,87 +241,26 @@ __device__ __forceinline__ bool same_hit(const Hit& a, const Hit& b) {
     return a.prim == b.prim && a.obj == b.obj && a.t == b.t;
 }
 
-// Closest hit of the ray (o, d) with the scene.  The triangles are tested in
-// a fixed order (ground, bottom, walls, top surface); a later hit replaces
-// the best one only when strictly closer.  Hit values come from mt_test
-// alone; the plane parameters, clipping and cell walk below only choose which
-// triangles to test, so they multiply by one reciprocal per axis instead of
-// dividing.
+// Closest hit of the ray (o, d) with the scene: among all triangles, the one
+// with the smallest t >= kTMin, ties going to the first in the order ground,
+// bottom, walls, top surface (the result of testing them one after another,
+// a hit replacing the best only when strictly closer).  Hit values come from
+// mt_test alone; the plane parameters, clipping and cell walk only choose
+// which triangles to test, so they multiply by one reciprocal per axis
+// instead of dividing.
+//
+// The top surface is walked first.  A plane (ground, bottom, a wall) whose
+// intersection parameter lies clearly beyond a hit already found -- by a
+// relative 1e-5, far above the ~1e-7 rounding of t -- cannot hold the
+// closest hit or tie with it, so its triangles are not tested; the result is
+// the same as testing everything.  (The bottom's and the walls' fallback to
+// neighbouring triangles compares coplanar triangles, whose t agree to
+// rounding, so it too is unaffected.)
 __device__ inline Hit trace(const SceneArgs& S, V3 o, V3 d) {
-    Hit best = miss_hit();
     const int R = S.R, e = R - 1;
     const PrimBase B = prim_base(R);
     const float rdx = 1.0f / d.x, rdy = 1.0f / d.y, rdz = 1.0f / d.z;  // inf on a zero component: never used then
-    // ground plane (2 triangles), skipped when it lies behind the ray (as for
-    // the bottom plane below: both triangles would fail t >= kTMin)
-    if (d.z != 0.0f && (S.sensor_z - o.z) * rdz >= 0.5f * kTMin) {
-        mt_test(o, d, v3(-1.f, -1.f, S.sensor_z), v3(1.f, -1.f, S.sensor_z), v3(-1.f, 1.f, S.sensor_z), 1, 0, best);
-        mt_test(o, d, v3(-1.f, 1.f, S.sensor_z), v3(1.f, -1.f, S.sensor_z), v3(1.f, 1.f, S.sensor_z), 1, 1, best);
-    }
-    // bottom plane z = 0: the two triangles of the cell under the plane hit
-    // (its 3x3 neighbourhood only if rounding put the hit across a cell edge).
-    // Every bottom triangle lies in that plane, so its intersection parameter
-    // is tp up to rounding (~1e-7 here): with tp < kTMin / 2 none of them can
-    // pass the t >= kTMin test -- the case of a ray leaving the bottom face,
-    // whose own cell and all 8 neighbours would otherwise be tested in vain.
-    if (d.z != 0.0f) {
-        const float tp = -o.z * rdz;
-        if (tp >= 0.5f * kTMin) {
-            const float x = fmaf(tp, d.x, o.x), y = fmaf(tp, d.y, o.y);
-            const float eps = 1e-5f;
-            if (x >= -1.0f - eps && x <= 1.0f + eps && y >= -1.0f - eps && y <= 1.0f + eps) {
-                const int c = clampi((int)floorf((x + 1.0f) * S.inv_delta), 0, e - 1);
-                const int r = clampi((int)floorf((y + 1.0f) * S.inv_delta), 0, e - 1);
-                const Hit keep = best;
-#pragma unroll 1
-                for (int j = 0; j < 9; ++j) {
-                    // j = 0: the cell itself; 1..8: its neighbours, row-major, if it held no hit
-                    if (j == 1 && !same_hit(best, keep)) break;
-                    int rr = r, cc = c;
-                    if (j > 0) {
-                        const int q = j - 1 + (j > 4 ? 1 : 0);
-                        rr = r + q / 3 - 1;
-                        cc = c + q % 3 - 1;
-                        if (rr < 0 || rr >= e || cc < 0 || cc >= e) continue;
-                    }
-                    const float x0 = coord(S, cc), x1 = coord(S, cc + 1), y0 = coord(S, rr), y1 = coord(S, rr + 1);
-                    quad_test(o, d, v3(x0, y0, 0.f), v3(x1, y0, 0.f), v3(x0, y1, 0.f), v3(x1, y1, 0.f), kBottomCode,
-                              B.bottom + 2 * (rr * e + cc), best);
-                }
-            }
-        }
-    }
-    // side walls x = -1, x = +1, y = -1, y = +1: the two triangles of the
-    // segment at the plane hit, when that hit lies within the wall's extent
-    // (the neighbouring segments if they miss)
-#pragma unroll 1
-    for (int w = 0; w < 4; ++w) {
-        const bool xw = w < 2;
-        const float dd = xw ? d.x : d.y;
-        if (dd == 0.0f) continue;
-        const float P = (w & 1) ? 1.0f : -1.0f;
-        const float tp = (P - (xw ? o.x : o.y)) * (xw ? rdx : rdy);
-        const float lat = xw ? fmaf(tp, d.y, o.y) : fmaf(tp, d.x, o.x);
-        const float z = fmaf(tp, d.z, o.z);
-        if (!(tp >= 0.0f) || !(z >= -1e-5f) || !(z <= S.hmax + 1e-5f) || !(fabsf(lat) <= 1.0f + 1e-5f)) continue;
-        const int k = clampi((int)floorf((lat + 1.0f) * S.inv_delta), 0, e - 1);
-        const int base = w == 0 ? B.left : (w == 1 ? B.right : (w == 2 ? B.front : B.back));
-        const int code = w == 0 ? kWallCode[0] : (w == 1 ? kWallCode[1] : (w == 2 ? kWallCode[2] : kWallCode[3]));
-        const int fixed = (w & 1) ? e : 0;  // the wall's grid column (x walls) or row (y walls)
-        const Hit keep = best;
-#pragma unroll 1
-        for (int j = 0; j < 3; ++j) {
-            // j = 0: the segment itself; then k - 1 and k + 1 if it held no hit
-            if (j == 1 && !same_hit(best, keep)) break;
-            const int kk = k + (j == 0 ? 0 : (j == 1 ? -1 : 1));
-            if (kk < 0 || kk >= e) continue;
-            const int ra = xw ? kk : fixed, ca = xw ? fixed : kk;
-            const int rb = xw ? kk + 1 : fixed, cb = xw ? fixed : kk + 1;
-            const float xa = coord(S, ca), ya = coord(S, ra), xb = coord(S, cb), yb = coord(S, rb);
-            quad_test(o, d, v3(xa, ya, 0.f), v3(xa, ya, height(S, ra, ca)), v3(xb, yb, 0.f),
-                      v3(xb, yb, height(S, rb, cb)), code, base + 2 * kk, best);
-        }
-    }
+    Hit top = miss_hit();
     // top surface: walk the cells under the ray's xy path while its z lies in
     // [0, hmax] (the surface's range); the first cell with a hit holds the
     // closest top hit.  Cells are visited in increasing t.
@@ -406,9 +345,95 @@ __device__ inline Hit trace(const SceneArgs& S, V3 o, V3 d) {
                 }
                 if (c < 0 || c >= e || r < 0 || r >= e) break;
             }
-            if (cell_best.obj >= 0 && (best.obj < 0 || cell_best.t < best.t)) best = cell_best;
+            top = cell_best;
         }
     }
+    // closest t found so far (any order), and whether a plane parameter is past it
+    float tk = top.obj >= 0 ? top.t : INFINITY;
+    auto beyond = [&](float tp) { return tp > fmaf(tk, 1e-5f, tk) + 1e-6f; };
+    // bottom plane z = 0: the two triangles of the cell under the plane hit
+    // (its 3x3 neighbourhood only if rounding put the hit across a cell edge).
+    // Every bottom triangle lies in that plane, so its intersection parameter
+    // is tp up to rounding (~1e-7 here): with tp < kTMin / 2 none of them can
+    // pass the t >= kTMin test -- the case of a ray leaving the bottom face,
+    // whose own cell and all 8 neighbours would otherwise be tested in vain.
+    Hit bot = miss_hit();
+    if (d.z != 0.0f) {
+        const float tp = -o.z * rdz;
+        if (tp >= 0.5f * kTMin && !beyond(tp)) {
+            const float x = fmaf(tp, d.x, o.x), y = fmaf(tp, d.y, o.y);
+            const float eps = 1e-5f;
+            if (x >= -1.0f - eps && x <= 1.0f + eps && y >= -1.0f - eps && y <= 1.0f + eps) {
+                const int c = clampi((int)floorf((x + 1.0f) * S.inv_delta), 0, e - 1);
+                const int r = clampi((int)floorf((y + 1.0f) * S.inv_delta), 0, e - 1);
+#pragma unroll 1
+                for (int j = 0; j < 9; ++j) {
+                    // j = 0: the cell itself; 1..8: its neighbours, row-major, if it held no hit
+                    if (j == 1 && bot.obj >= 0) break;
+                    int rr = r, cc = c;
+                    if (j > 0) {
+                        const int q = j - 1 + (j > 4 ? 1 : 0);
+                        rr = r + q / 3 - 1;
+                        cc = c + q % 3 - 1;
+                        if (rr < 0 || rr >= e || cc < 0 || cc >= e) continue;
+                    }
+                    const float x0 = coord(S, cc), x1 = coord(S, cc + 1), y0 = coord(S, rr), y1 = coord(S, rr + 1);
+                    quad_test(o, d, v3(x0, y0, 0.f), v3(x1, y0, 0.f), v3(x0, y1, 0.f), v3(x1, y1, 0.f), kBottomCode,
+                              B.bottom + 2 * (rr * e + cc), bot);
+                }
+            }
+        }
+    }
+    if (bot.obj >= 0) tk = fminf(tk, bot.t);
+    // ground plane (2 triangles), skipped when it lies behind the ray (as for
+    // the bottom plane: both triangles would fail t >= kTMin) or past tk
+    Hit best = miss_hit();
+    if (d.z != 0.0f) {
+        const float tg = (S.sensor_z - o.z) * rdz;
+        if (tg >= 0.5f * kTMin && !beyond(tg)) {
+            mt_test(o, d, v3(-1.f, -1.f, S.sensor_z), v3(1.f, -1.f, S.sensor_z), v3(-1.f, 1.f, S.sensor_z), 1, 0,
+                    best);
+            mt_test(o, d, v3(-1.f, 1.f, S.sensor_z), v3(1.f, -1.f, S.sensor_z), v3(1.f, 1.f, S.sensor_z), 1, 1, best);
+        }
+    }
+    // in test order: ground, then bottom
+    if (bot.obj >= 0 && (best.obj < 0 || bot.t < best.t)) best = bot;
+    if (best.obj >= 0) tk = fminf(tk, best.t);
+    // side walls x = -1, x = +1, y = -1, y = +1: the two triangles of the
+    // segment at the plane hit, when that hit lies within the wall's extent
+    // (the neighbouring segments if they miss)
+#pragma unroll 1
+    for (int w = 0; w < 4; ++w) {
+        const bool xw = w < 2;
+        const float dd = xw ? d.x : d.y;
+        if (dd == 0.0f) continue;
+        const float P = (w & 1) ? 1.0f : -1.0f;
+        const float tp = (P - (xw ? o.x : o.y)) * (xw ? rdx : rdy);
+        const float lat = xw ? fmaf(tp, d.y, o.y) : fmaf(tp, d.x, o.x);
+        const float z = fmaf(tp, d.z, o.z);
+        if (!(tp >= 0.0f) || !(z >= -1e-5f) || !(z <= S.hmax + 1e-5f) || !(fabsf(lat) <= 1.0f + 1e-5f)) continue;
+        if (beyond(tp)) continue;
+        const int k = clampi((int)floorf((lat + 1.0f) * S.inv_delta), 0, e - 1);
+        const int base = w == 0 ? B.left : (w == 1 ? B.right : (w == 2 ? B.front : B.back));
+        const int code = w == 0 ? kWallCode[0] : (w == 1 ? kWallCode[1] : (w == 2 ? kWallCode[2] : kWallCode[3]));
+        const int fixed = (w & 1) ? e : 0;  // the wall's grid column (x walls) or row (y walls)
+        const Hit keep = best;
+#pragma unroll 1
+        for (int j = 0; j < 3; ++j) {
+            // j = 0: the segment itself; then k - 1 and k + 1 if it held no hit
+            if (j == 1 && !same_hit(best, keep)) break;
+            const int kk = k + (j == 0 ? 0 : (j == 1 ? -1 : 1));
+            if (kk < 0 || kk >= e) continue;
+            const int ra = xw ? kk : fixed, ca = xw ? fixed : kk;
+            const int rb = xw ? kk + 1 : fixed, cb = xw ? fixed : kk + 1;
+            const float xa = coord(S, ca), ya = coord(S, ra), xb = coord(S, cb), yb = coord(S, rb);
+            quad_test(o, d, v3(xa, ya, 0.f), v3(xa, ya, height(S, ra, ca)), v3(xb, yb, 0.f),
+                      v3(xb, yb, height(S, rb, cb)), code, base + 2 * kk, best);
+        }
+        if (best.obj >= 0) tk = fminf(tk, best.t);
+    }
+    // the top surface comes last in test order
+    if (top.obj >= 0 && (best.obj < 0 || top.t < best.t)) best = top;
     return best;
 }
 
