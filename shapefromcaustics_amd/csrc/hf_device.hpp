@@ -23,7 +23,7 @@
 namespace hf {
 
 constexpr float kTMin = 1e-3f;
-constexpr float kCullZ = 1e-3f;  // height margin of the cell cull (float rounding of z is ~1e-7 here)
+constexpr float kCullZ = 1e-3f;  // relative height margin of the cell cull (float rounding of z: ~1e-7 relative)
 constexpr float kMissT = -2.0f;
 constexpr int kMaxBounces = 8;
 
@@ -318,7 +318,7 @@ __device__ inline Hit trace(const SceneArgs& S, V3 o, V3 d) {
                 // Cull: a hit on the cell's triangles lies over the cell, i.e.
                 // at t in [tcur, texit], and between the cell's lowest and
                 // highest vertex; if the ray's z over that interval stays
-                // clear of those heights (by far more than rounding) the
+                // clear of those heights (by 1e-3 relative, far more than rounding) the
                 // triangles cannot be hit.  Rays inside the glass, below the
                 // surface, skip most cells this way.
                 bool test = true;
@@ -326,7 +326,8 @@ __device__ inline Hit trace(const SceneArgs& S, V3 o, V3 d) {
                     const float za = fmaf(tcur, d.z, o.z), zb = d.z != 0.0f ? fmaf(texit, d.z, o.z) : o.z;
                     const float cmin = fminf(fminf(h00, h01), fminf(h10, h11));
                     const float cmax = fmaxf(fmaxf(h00, h01), fmaxf(h10, h11));
-                    test = fmaxf(za, zb) >= cmin - kCullZ && fminf(za, zb) <= cmax + kCullZ;
+                    const float m = kCullZ * (1.0f + fmaxf(fabsf(za), fabsf(zb)));
+                    test = fmaxf(za, zb) >= cmin - m && fminf(za, zb) <= cmax + m;
                 }
                 if (test) {
                     const float x0 = coord(S, c), x1 = coord(S, c + 1), y0 = coord(S, r), y1 = coord(S, r + 1);
