@@ -35,7 +35,7 @@ _lock = threading.Lock()
 
 TILES = (8, 16, 32, 64)
 SEGS = (512, 1024, 2048)
-BLOCKS = (2048, 4096, 8192)
+BLOCKS = (1024, 2048, 4096, 8192)
 HEADROOM = 1.25  # capacity = measured records x HEADROOM (+ 5 per 100 photons)
 
 
