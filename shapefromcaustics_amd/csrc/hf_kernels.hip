@@ -258,14 +258,21 @@ struct Refr {
     bool flip, tir, clamped;
 };
 
-__device__ __forceinline__ Refr refr_setup(V3 d, V3 ns, float eo, float ei) {
+// The two index ratios of a photon's path, eo / ei (entering the glass) and
+// ei / eo (leaving it), divided once per photon instead of at every bounce.
+struct Etas {
+    float enter, exit;
+};
+__device__ __forceinline__ Etas etas_of(float eo, float ei) { return Etas{eo / ei, ei / eo}; }
+
+__device__ __forceinline__ Refr refr_setup(V3 d, V3 ns, const Etas& et) {
     Refr R;
     const float raw = dot(d, ns);
     R.clamped = !(raw >= -1.0f && raw <= 1.0f);
     R.cosi = fminf(fmaxf(raw, -1.0f), 1.0f);
     R.flip = R.cosi > 0.0f;
     R.N = R.flip ? -ns : ns;
-    R.eta = R.flip ? ei / eo : eo / ei;
+    R.eta = R.flip ? et.exit : et.enter;
     const float k = 1.0f - R.eta * R.eta * (1.0f - R.cosi * R.cosi);
     R.tir = !(k > 0.0f);
     R.sq = R.tir ? 0.0f : sqrtf(k);
@@ -351,7 +358,7 @@ __global__ __launch_bounds__(256) void simulate_kernel(SceneArgs S, const float*
     S.hmax = hmax[0];
     int ch;
     PathState s = emit(E, i, ch);
-    const float ei = ior_of(E, ch), eo = 1.0f;
+    const Etas et = etas_of(1.0f, ior_of(E, ch));  // outside: index 1
     int nb = 0;
     for (int bounce = 0; bounce < kMaxBounces; ++bounce) {
         const Hit hit = trace(S, s.o, s.d);
@@ -383,7 +390,7 @@ __global__ __launch_bounds__(256) void simulate_kernel(SceneArgs S, const float*
             if (nhit) nhit[i] = nb <= kHitSlots ? (uint8_t)nb : kHitsRetrace;
             return;
         }
-        const Refr R = refr_setup(s.d, ns, eo, ei);
+        const Refr R = refr_setup(s.d, ns, et);
         refract_apply(R, s.d, s.Dt, s.Dp);
         s.Du = Du;
         s.Dv = Dv;
@@ -405,14 +412,14 @@ struct Tape {
 // One bounce of the forward path after its hit: move to the hit point,
 // transfer the differentials and refract -- the update of simulate_kernel,
 // operation for operation, so a replay from recorded hits is bit-identical.
-__device__ __forceinline__ void advance(const SceneArgs& S, PathState& s, const Hit& hit, float eo, float ei) {
+__device__ __forceinline__ void advance(const SceneArgs& S, PathState& s, const Hit& hit, const Etas& et) {
     const V3 p = s.o + hit.t * s.d;
     const TriVerts T = tri_verts(S, hit.obj, hit.prim);
     const V3 ng = geometric_normal(T);
     const V3 ns = normalize(shading_raw(T, hit.b1, hit.b2));
     const V3 Du = transfer1(ng, s.d, hit.t, s.Du, s.Dt);
     const V3 Dv = transfer1(ng, s.d, hit.t, s.Dv, s.Dp);
-    const Refr R = refr_setup(s.d, ns, eo, ei);
+    const Refr R = refr_setup(s.d, ns, et);
     refract_apply(R, s.d, s.Dt, s.Dp);
     s.Du = Du;
     s.Dv = Dv;
@@ -494,7 +501,7 @@ __device__ __forceinline__ void adjoint_photon(const SceneArgs& S, const EmitArg
     const bool replay = m != kHitsRetrace;
     int ch;
     const PathState s0 = emit(E, i, ch);
-    const float ei = ior_of(E, ch), eo = 1.0f;
+    const Etas et = etas_of(1.0f, ior_of(E, ch));  // outside: index 1
     auto hit_at = [&](int b) { return unpack_hit(hits[(int64_t)b * N + i]); };
     Tape tape[kMaxBounces];  // traced paths only
     int nb = 0;
@@ -502,7 +509,7 @@ __device__ __forceinline__ void adjoint_photon(const SceneArgs& S, const EmitArg
     if (replay) {
         nb = m;  // the last recorded hit is the ground
         for (int b = 0; b + 1 < m; ++b) {
-            advance(S, s, hit_at(b), eo, ei);
+            advance(S, s, hit_at(b), et);
             if (b == 0) s1 = s;
         }
     } else {
@@ -523,7 +530,7 @@ __device__ __forceinline__ void adjoint_photon(const SceneArgs& S, const EmitArg
                 reached = sp.good;
                 break;
             }
-            advance(S, s, hit, eo, ei);
+            advance(S, s, hit, et);
         }
         if (!reached) return;
     }
@@ -533,7 +540,7 @@ __device__ __forceinline__ void adjoint_photon(const SceneArgs& S, const EmitArg
         if (k == nb - 1) return tape_of(s, hit_at(k));
         if (k == 0) return tape_of(s0, hit_at(0));
         PathState r = s1;
-        for (int b = 1; b < k; ++b) advance(S, r, hit_at(b), eo, ei);
+        for (int b = 1; b < k; ++b) advance(S, r, hit_at(b), et);
         return tape_of(r, hit_at(k));
     };
     // upstream gradients of the splat inputs
@@ -583,7 +590,7 @@ __device__ __forceinline__ void adjoint_photon(const SceneArgs& S, const EmitArg
         const V3 m = shading_raw(T, B.b1, B.b2);
         const float mlen = norm(m);
         const V3 ns = mlen > 0.0f ? (1.0f / mlen) * m : v3(0.f, 0.f, 0.f);
-        const Refr R = refr_setup(B.d, ns, eo, ei);
+        const Refr R = refr_setup(B.d, ns, et);
         // refraction adjoint: outputs d', Dt', Dp' (gradients gd, gDt, gDp)
         V3 gd_in = v3(0.f, 0.f, 0.f), gDt_in = gd_in, gDp_in = gd_in, gN = gd_in;
         float gcosi = 0.0f;

@@ -13,6 +13,9 @@
 
 #define __device__
 #define __forceinline__ inline
+// the tracer's hardware reciprocal (1 ulp on the GPU) as a host division: it
+// only steers which triangles are tested, never a hit's values
+#define __builtin_amdgcn_rcpf(x) (1.0f / (x))
 #include "hf_device_host.hpp"
 
 using namespace hf;
