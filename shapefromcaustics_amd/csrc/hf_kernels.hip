@@ -752,7 +752,8 @@ __global__ __launch_bounds__(256) void grad_h_kernel(int R, const float* __restr
 // ---------------------------------------------------------------------------
 // Landweber data term and update
 // ---------------------------------------------------------------------------
-constexpr int kRedBlocks = 256;
+constexpr int kRedBlocks = 256;    // workgroups of the Landweber update's volume sum
+constexpr int kResidBlocks = 1024;  // workgroups of the residual (enough loads in flight for 1024^2 pixels)
 
 __global__ __launch_bounds__(256) void residual_kernel(const float* __restrict__ img, const float* __restrict__ tgt,
                                                        int64_t n, float* __restrict__ resid,
@@ -814,14 +815,15 @@ __global__ __launch_bounds__(256) void landweber_step_kernel(const float* __rest
 __global__ __launch_bounds__(256) void vcons_kernel(const float* __restrict__ x, const float* __restrict__ mask,
                                                     int R, LwArgs A, const float* __restrict__ partial, int nparts,
                                                     float* __restrict__ d) {
-    __shared__ float vx_s;
-    if (threadIdx.x == 0) {
-        float acc = 0.0f;
-        for (int k = 0; k < nparts; ++k) acc += partial[k];
-        vx_s = acc;
-    }
+    // the volume: every workgroup reduces the partials in the same fixed
+    // order, so all of them see the same value
+    __shared__ float part[4];
+    float acc = 0.0f;
+    for (int k = threadIdx.x; k < nparts; k += 256) acc += partial[k];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
     __syncthreads();
-    const float vx = vx_s;
+    const float vx = (part[0] + part[1]) + (part[2] + part[3]);
     const int k = blockIdx.x * 256 + threadIdx.x;
     if (k >= R * R) return;
     const int r = k / R, c = k - r * R;
@@ -962,8 +964,9 @@ int adjoint_impl(const float* h, const float* normals, const hf_scene* sc, const
     const float4* hrec = want_hits ? (const float4*)((const char*)hits + HL.recs) : nullptr;
     const uint8_t* hcnt = want_hits ? (const uint8_t*)hits + HL.count : nullptr;
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(zero_kernel, dim3(blocks_for(3 * RR)), dim3(256), 0, s, gn, 3 * RR);
-    hipLaunchKernelGGL(zero_kernel, dim3(blocks_for(RR)), dim3(256), 0, s, gz, RR);
+    // grad_n and grad_z are adjacent in the workspace: one fill clears both (and the padding between)
+    const int64_t nz = (int64_t)((L.grad_z - L.grad_n) / sizeof(float)) + RR;
+    hipLaunchKernelGGL(zero_kernel, dim3(blocks_for(nz)), dim3(256), 0, s, gn, nz);
     launch_hmax(h, RR, hmax, s);
     const int tiles_x = (em->n_side + 15) / 16;
     hipLaunchKernelGGL(adjoint_kernel, dim3((unsigned)(tiles_x * tiles_x), (unsigned)sc->channels), dim3(256), 0, s,
@@ -1046,17 +1049,17 @@ int hf_adjoint_hits_f32(const float* h, const float* normals, const hf_scene* sc
                         stream);
 }
 
-size_t hf_residual_workspace_bytes(int64_t n) { return n < 0 ? 0 : sizeof(float) * kRedBlocks; }
+size_t hf_residual_workspace_bytes(int64_t n) { return n < 0 ? 0 : sizeof(float) * kResidBlocks; }
 
 int hf_residual_f32(const float* img, const float* target, int64_t n, float* resid, float* loss, void* ws,
                     size_t ws_bytes, void* stream) {
     g_err[0] = '\0';
     if (n < 0 || (n > 0 && (!img || !target || !resid)) || !loss) return fail(PDS_ERR_ARGUMENT, "invalid residual args");
-    if (!ws || ws_bytes < sizeof(float) * kRedBlocks) return fail(PDS_ERR_WORKSPACE, "residual workspace too small");
+    if (!ws || ws_bytes < sizeof(float) * kResidBlocks) return fail(PDS_ERR_WORKSPACE, "residual workspace too small");
     hipStream_t s = (hipStream_t)stream;
     float* partial = (float*)ws;
-    hipLaunchKernelGGL(residual_kernel, dim3(kRedBlocks), dim3(256), 0, s, img, target, n, resid, partial);
-    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, s, (const float*)partial, kRedBlocks, 0.5f, loss);
+    hipLaunchKernelGGL(residual_kernel, dim3(kResidBlocks), dim3(256), 0, s, img, target, n, resid, partial);
+    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, s, (const float*)partial, kResidBlocks, 0.5f, loss);
     return hip_status("hf_residual");
 }
 
