@@ -28,6 +28,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 #   forward : reads E 4 + xp 8 + M 24 + cp 8 + radius 4 = 48 B per photon, writes 4 B per pixel
 #   backward: reads the same 48 B, writes gE 4 + gx 8 + gM 24 = 36 B per photon, reads 4 B per pixel
 FWD_B_PER_PHOTON, BWD_B_PER_PHOTON, B_PER_PIXEL_PASS = 48, 84, 4
+EVENT_EVERY = 4  # timed steps per step that carries HIP events (see timed())
 
 
 def log(*a):
@@ -357,22 +358,27 @@ def main():
             ev[2].record()
         return img, g
 
-    def timed(run, steps, warmup, split=True):
-        """run(ev) does one step, recording ev[0..2] (start, after forward, end)."""
+    def timed(run, steps, warmup, split=True, every=EVENT_EVERY):
+        """run(ev) does one step, recording ev[0..2] (start, after forward, end)
+        when ev is not None.  Every `every`-th timed step carries the three
+        HIP events (each event record costs a few microseconds of stream
+        time, so instrumenting every step would inflate the wall time); the
+        pass durations are the medians over those steps."""
         for _ in range(warmup):
             run(None)
-        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+        marked = set(range(0, steps, every))
+        evs = {k: [torch.cuda.Event(enable_timing=True) for _ in range(3)] for k in marked}
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for k in range(steps):
-            run(evs[k])
+            run(evs.get(k))
         torch.cuda.synchronize()
         wall = (time.perf_counter() - t0) / steps
         if not split:
-            tot = sorted(e[0].elapsed_time(e[2]) for e in evs)
+            tot = sorted(e[0].elapsed_time(e[2]) for e in evs.values())
             return wall, tot[len(tot) // 2] * 1e-3, None
-        fwd = sorted(e[0].elapsed_time(e[1]) for e in evs)
-        bwd = sorted(e[1].elapsed_time(e[2]) for e in evs)
+        fwd = sorted(e[0].elapsed_time(e[1]) for e in evs.values())
+        bwd = sorted(e[1].elapsed_time(e[2]) for e in evs.values())
         return wall, fwd[len(fwd) // 2] * 1e-3, bwd[len(bwd) // 2] * 1e-3
 
     tuning = "cached" if args.tuned else None
