@@ -465,6 +465,10 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(kname, args.workload),
                 "kernel": kname, "algorithmic_bytes": kbytes, "avg_duration_us": round(kt * 1e6, 2),
                 "step_achieved_GBps": round((fwd_bytes + bwd_bytes) / wall / 1e9, 1)}
+    if roofline["traffic"]:
+        # the bytes rocprof counted for this pass (profiles/), moved in the pass time measured here
+        roofline["traffic_GBps"] = round(roofline["traffic"] / kt / 1e9, 1)
+        roofline["traffic_frac"] = round(roofline["traffic"] / kt / 1e9 / HBM_PEAK_GBS, 4)
 
     cpu = None
     if not args.no_cpu:
