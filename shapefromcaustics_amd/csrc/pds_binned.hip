@@ -625,6 +625,7 @@ __device__ __forceinline__ int chord_pixels(const TileRec& t) {
 struct ChordWalk {
     int r, kx, xb, li;
     float dy;
+    __device__ __forceinline__ float dx(float fx) const { return (float)kx - fx; }
     // move to the first pixel of the first non-empty chord at or after row r
     __device__ __forceinline__ void seek(const TileRec& t, int tshift) {
         int xa = 1;
@@ -660,24 +661,30 @@ struct ChordWalk {
 // order, dx, dy and LDS index as ChordWalk over a full box, with a
 // branch-free row advance (a divergent seek would cost every lane of the
 // wavefront whenever one lane changes row, which is most steps).
+//
+// The pixel offsets (kx, ky) are counted in float: they are small integers,
+// exact in float, so (kx - fx) is the same float as ((float)kx - fx) and the
+// walk spends no int-to-float conversions per pixel.
 struct BoxWalk {
-    int kx, ky, li, kx0, xb, skip;
+    float kx, ky, kx0, xb;
+    int li, skip;
     float dy;
+    __device__ __forceinline__ float dx(float fx) const { return kx - fx; }
     __device__ __forceinline__ void start(const TileRec& t, int tshift) {
-        kx0 = t.kx0;
-        kx = t.kx0;
-        ky = t.ky0;
-        xb = t.kx0 + t.bw - 1;
+        kx0 = (float)t.kx0;
+        kx = kx0;
+        ky = (float)t.ky0;
+        xb = (float)(t.kx0 + t.bw - 1);
         li = (t.ly0 << tshift) + t.lx0;
         skip = (1 << tshift) - t.bw + 1;
-        dy = (float)ky - t.f.fy;
+        dy = ky - t.f.fy;
     }
     __device__ __forceinline__ void step(const TileRec& t, int, bool) {
         const bool nl = kx >= xb;
-        kx = nl ? kx0 : kx + 1;
+        kx = nl ? kx0 : kx + 1.0f;
         li += nl ? skip : 1;
-        ky += nl ? 1 : 0;
-        dy = nl ? (float)ky - t.f.fy : dy;
+        ky += nl ? 1.0f : 0.0f;
+        dy = ky - t.f.fy;
     }
 };
 
@@ -963,7 +970,7 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_kernel(Geom g, PhotonAr
         const int n = key < 255 ? key : chord_pixels(t);
         auto walk = [&](auto& w) {
             for (int p = 0; p < n; ++p) {
-                const float dx = (float)w.kx - t.f.fx;
+                const float dx = w.dx(t.f.fx);
                 float u, v;
                 const float val = silverman(eval_q(t.f, dx, w.dy, u, v)) * t.e;
                 if (val > 0.0f) {
@@ -1114,7 +1121,7 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_bwd_kernel(Geom g, Imag
             float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f, m4 = 0.f, m5 = 0.f;
             auto walk = [&](auto& cw) {
                 for (int p = 0; p < n; ++p) {
-                    const float dx = (float)cw.kx - t.f.fx, dy = cw.dy;
+                    const float dx = cw.dx(t.f.fx), dy = cw.dy;
                     float u, v;
                     const float q = eval_q(t.f, dx, dy, u, v);
                     const float w = q < 1.0f ? 1.0f - q : 0.0f;
