@@ -630,15 +630,19 @@ __device__ __forceinline__ void adjoint_photon(const SceneArgs& S, const EmitArg
         transfer1_adj(ng, B.d, B.t, B.Dv, B.Dp, gDv, gDv_in, gDp_in, gd_in, gng);
         // p = o + t d: gradient of o is gp (passed on), of d is t gp
         gd_in = gd_in + B.t * gp;
-        // shading normal -> vertex normals (top-surface vertices only)
-        if (mlen > 0.0f) {
+        // shading normal -> vertex normals (top-surface triangles only: the
+        // other faces' vertex normals are constants)
+        const bool top_face = B.prim < prim_base(S.R).bottom;
+        if (top_face && mlen > 0.0f) {
             const V3 gm = (1.0f / mlen) * (gns - dot(gns, ns) * ns);
             const float w[3] = {w0, B.b1, B.b2};
             for (int v = 0; v < 3; ++v)
-                if (T.vr[v] >= 0 && B.prim < prim_base(S.R).bottom) A.add_n(T.vr[v], T.vc[v], w[v] * gm);
+                if (T.vr[v] >= 0) A.add_n(T.vr[v], T.vc[v], w[v] * gm);
         }
-        // geometric normal -> vertex heights
-        if (glen > 0.0f) {
+        // geometric normal -> vertex heights (vertices whose height is h: the
+        // top surface's and the walls' upper ones; none on the bottom face)
+        const bool any_h = T.vr[0] >= 0 || T.vr[1] >= 0 || T.vr[2] >= 0;
+        if (any_h && glen > 0.0f) {
             const V3 gg = (1.0f / glen) * (gng - dot(gng, ng) * ng);
             const V3 e1 = T.v[1] - T.v[0], e2 = T.v[2] - T.v[0];
             const V3 ge1 = cross(e2, gg), ge2 = cross(gg, e1);
