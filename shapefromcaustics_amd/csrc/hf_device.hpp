@@ -404,9 +404,30 @@ __device__ inline Hit trace(const SceneArgs& S, V3 o, V3 d) {
     if (best.obj >= 0) tk = fminf(tk, best.t);
     // side walls x = -1, x = +1, y = -1, y = +1: the two triangles of the
     // segment at the plane hit, when that hit lies within the wall's extent
-    // (the neighbouring segments if they miss)
+    // (the neighbouring segments if they miss).  A wall hit needs the ray to
+    // cross |x| = 1 or |y| = 1 at a height within the slab; if the ray's xy
+    // stays inside the square at both ends of its slab interval (widened by
+    // 1e-3 in z, 1e-4 in xy: far above rounding) it stays inside in between,
+    // and the four walls are not tested.
+    bool walls = true;
+    if (d.z != 0.0f) {
+        float ta = (-1e-3f - o.z) * rdz, tb = (S.hmax + 1e-3f - o.z) * rdz;
+        if (ta > tb) {
+            const float tmp = ta;
+            ta = tb;
+            tb = tmp;
+        }
+        ta = fmaxf(ta, 0.0f);
+        if (!(ta <= tb)) {
+            walls = false;  // the slab lies behind the ray
+        } else {
+            const float lim = 1.0f - 1e-4f;
+            walls = !(fabsf(fmaf(ta, d.x, o.x)) < lim && fabsf(fmaf(ta, d.y, o.y)) < lim &&
+                      fabsf(fmaf(tb, d.x, o.x)) < lim && fabsf(fmaf(tb, d.y, o.y)) < lim);
+        }
+    }
 #pragma unroll 1
-    for (int w = 0; w < 4; ++w) {
+    for (int w = 0; w < (walls ? 4 : 0); ++w) {
         const bool xw = w < 2;
         const float dd = xw ? d.x : d.y;
         if (dd == 0.0f) continue;
