@@ -115,7 +115,9 @@ int hf_adjoint_f32(const float* h, const float* normals, const hf_scene* scene, 
  * photon that reaches the sensor, its first 4 hits (t, barycentrics,
  * object, primitive) in `hits` (hf_hits_bytes), and hf_adjoint_hits_f32
  * replays those paths without tracing them again (paths with more bounces
- * are traced again).  Every photon's gradient terms are those of
+ * are traced again).  The adjoint takes `hits` from a simulate on the same
+ * h, scene and emitter: the buffer also carries that field's maximum height,
+ * which the re-traced paths use.  Every photon's gradient terms are those of
  * hf_adjoint_f32 bit for bit; dL/dh differs only by the (run-dependent)
  * order of the float atomic sums. */
 size_t hf_hits_bytes(const hf_scene* scene, const hf_emitter* emitter);

@@ -97,7 +97,9 @@ __global__ __launch_bounds__(256) void hmax_part_kernel(const float* __restrict_
     if (threadIdx.x == 0) part[blockIdx.x] = fmaxf(fmaxf(wpart[0], wpart[1]), fmaxf(wpart[2], wpart[3]));
 }
 
-__global__ __launch_bounds__(1024) void hmax_kernel(const float* __restrict__ h, int64_t n, float* __restrict__ out) {
+// out[0] = the maximum (and copy[0] too, when copy is not null)
+__global__ __launch_bounds__(1024) void hmax_kernel(const float* __restrict__ h, int64_t n, float* __restrict__ out,
+                                                    float* __restrict__ copy) {
     __shared__ float part[16];
     float m = -INFINITY;
     for (int64_t k = threadIdx.x; k < n; k += 1024) m = fmaxf(m, h[k]);
@@ -108,6 +110,7 @@ __global__ __launch_bounds__(1024) void hmax_kernel(const float* __restrict__ h,
         float r = -INFINITY;
         for (int w = 0; w < 16; ++w) r = fmaxf(r, part[w]);
         out[0] = r;
+        if (copy) copy[0] = r;
     }
 }
 
@@ -852,10 +855,12 @@ __global__ __launch_bounds__(256) void vcons_kernel(const float* __restrict__ x,
 
 unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
 
-// hmax[0] = max of the n heights (hmax: the workspace's hmax region)
-void launch_hmax(const float* h, int64_t n, float* hmax, hipStream_t s) {
+// hmax[0] = max of the n heights (hmax: the workspace's hmax region); also
+// written to copy[0] when copy is not null
+void launch_hmax(const float* h, int64_t n, float* hmax, hipStream_t s, float* copy = nullptr) {
     hipLaunchKernelGGL(hmax_part_kernel, dim3(kHmaxBlocks), dim3(256), 0, s, h, n, hmax + 64);
-    hipLaunchKernelGGL(hmax_kernel, dim3(1), dim3(1024), 0, s, (const float*)(hmax + 64), (int64_t)kHmaxBlocks, hmax);
+    hipLaunchKernelGGL(hmax_kernel, dim3(1), dim3(1024), 0, s, (const float*)(hmax + 64), (int64_t)kHmaxBlocks, hmax,
+                       copy);
 }
 
 EmitArgs emit_args(const hf_scene* sc, const hf_emitter* em) {
@@ -908,15 +913,19 @@ int hip_status(const char* what) {
     return PDS_OK;
 }
 
+// hit buffer: records [kHitSlots][N] float4, counts [N] bytes, and the
+// maximum height of the field the records were traced on (the adjoint's
+// re-traced paths use it, so it is not computed again)
 struct HitsLayout {
-    size_t recs, count, end;
+    size_t recs, count, hmax, end;
 };
 
 HitsLayout hits_layout(int64_t N) {
     HitsLayout L;
     L.recs = 0;
     L.count = (sizeof(float4) * kHitSlots * (size_t)N + 255) & ~(size_t)255;
-    L.end = L.count + (size_t)N;
+    L.hmax = (L.count + (size_t)N + 255) & ~(size_t)255;
+    L.end = L.hmax + sizeof(float);
     return L;
 }
 
@@ -939,7 +948,7 @@ int simulate_impl(const float* h, const float* normals, const hf_scene* sc, cons
     float4* hrec = want_hits ? (float4*)((char*)hits + HL.recs) : nullptr;
     uint8_t* hcnt = want_hits ? (uint8_t*)hits + HL.count : nullptr;
     hipStream_t s = (hipStream_t)stream;
-    launch_hmax(h, (int64_t)sc->R * sc->R, hmax, s);
+    launch_hmax(h, (int64_t)sc->R * sc->R, hmax, s, want_hits ? (float*)((char*)hits + HL.hmax) : nullptr);
     hipLaunchKernelGGL(simulate_kernel, dim3(blocks_for(N)), dim3(256), 0, s, scene_args(h, normals, sc, hmax),
                        (const float*)hmax, emit_args(sc, em), O, hrec, hcnt);
     return hip_status("hf_simulate");
@@ -973,11 +982,16 @@ int adjoint_impl(const float* h, const float* normals, const hf_scene* sc, const
     // grad_n and grad_z are adjacent in the workspace: one fill clears both (and the padding between)
     const int64_t nz = (int64_t)((L.grad_z - L.grad_n) / sizeof(float)) + RR;
     hipLaunchKernelGGL(zero_kernel, dim3(blocks_for(nz)), dim3(256), 0, s, gn, nz);
-    launch_hmax(h, RR, hmax, s);
+    // with hit records, the maximum height the simulate stored beside them
+    const float* hm = hmax;
+    if (want_hits)
+        hm = (const float*)((const char*)hits + HL.hmax);
+    else
+        launch_hmax(h, RR, hmax, s);
     const int tiles_x = (em->n_side + 15) / 16;
     hipLaunchKernelGGL(adjoint_kernel, dim3((unsigned)(tiles_x * tiles_x), (unsigned)sc->channels), dim3(256), 0, s,
-                       scene_args(h, normals, sc, hmax), (const float*)hmax, emit_args(sc, em), N, tiles_x, grad_Ep,
-                       grad_xp, grad_Mp, gn, gz, hrec, hcnt);
+                       scene_args(h, normals, sc, hm), hm, emit_args(sc, em), N, tiles_x, grad_Ep, grad_xp, grad_Mp, gn,
+                       gz, hrec, hcnt);
     hipLaunchKernelGGL(normals_adj_kernel, dim3(blocks_for(RR)), dim3(256), 0, s, h, R, s_sobel, (const float*)gn, gg);
     hipLaunchKernelGGL(grad_h_kernel, dim3(blocks_for(RR)), dim3(256), 0, s, R, (const float*)gg, (const float*)gz,
                        grad_h);

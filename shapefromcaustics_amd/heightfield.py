@@ -210,6 +210,16 @@ def trace_rays(h: torch.Tensor, scene: Scene, origins: torch.Tensor, dirs: torch
     return t, uv, obj, prim
 
 
+def hit_counts(hits: torch.Tensor, n_photons: int) -> torch.Tensor:
+    """The per-photon hit counts inside a hit buffer from ``simulate(...,
+    keep_hits=True)``: 0 = dropped, 1..4 = hits recorded, 255 = more bounces
+    than recorded (the adjoint traces that path again).  Layout of the buffer
+    (csrc/hf_kernels.hip, hits_layout): 4 x N float4 records, then the N count
+    bytes at the next 256-byte boundary, then the field's maximum height."""
+    off = (16 * 4 * n_photons + 255) // 256 * 256
+    return hits[off:off + n_photons]
+
+
 def simulate(h: torch.Tensor, scene: Scene, emitter: ParallelEmitter, n: tp.Optional[torch.Tensor] = None,
              keep_hits: bool = False):
     """Splat inputs (Ep, xp, Mp, cp, radius) of every emitted photon; dropped photons have cp = -1.
