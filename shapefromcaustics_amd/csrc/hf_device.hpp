@@ -54,7 +54,8 @@ struct SceneArgs {
     float delta;           // 2 / (R - 1)
     float inv_delta;
     float sensor_z;        // ground plane height
-    float hmax;            // max(h) (top-surface traversal bound)
+    float hmax;            // max(h), min(h): the top surface's z range (traversal bounds)
+    float hmin;
 };
 
 __device__ __forceinline__ float height(const SceneArgs& S, int r, int c) { return S.h[r * S.R + c]; }
@@ -264,8 +265,8 @@ __device__ inline Hit trace(const SceneArgs& S, V3 o, V3 d) {
     const float rdx = __builtin_amdgcn_rcpf(d.x), rdy = __builtin_amdgcn_rcpf(d.y), rdz = __builtin_amdgcn_rcpf(d.z);
     Hit top = miss_hit();
     // top surface: walk the cells under the ray's xy path while its z lies in
-    // [0, hmax] (the surface's range); the first cell with a hit holds the
-    // closest top hit.  Cells are visited in increasing t.
+    // the surface's range [hmin, hmax] (widened below); the first cell with a
+    // hit holds the closest top hit.  Cells are visited in increasing t.
     {
         float t0 = 0.0f, t1 = INFINITY;
         // clip to the square [-1, 1]^2
@@ -285,11 +286,14 @@ __device__ inline Hit trace(const SceneArgs& S, V3 o, V3 d) {
         };
         clip(o.x, d.x, rdx);
         clip(o.y, d.y, rdy);
-        // clip to the slab 0 <= z <= hmax
+        // clip to the slab zlo <= z <= hmax: the top surface lies within
+        // [hmin, hmax] (zlo = hmin less 1e-3 relative, far above rounding;
+        // never below the bottom face's 0)
+        const float zlo = fmaxf(0.0f, S.hmin - kCullZ * (1.0f + fabsf(S.hmin)));
         if (d.z == 0.0f) {
-            if (o.z < 0.0f || o.z > S.hmax) t1 = -1.0f;
+            if (o.z < zlo || o.z > S.hmax) t1 = -1.0f;
         } else {
-            float a = (0.0f - o.z) * rdz, b = (S.hmax - o.z) * rdz;
+            float a = (zlo - o.z) * rdz, b = (S.hmax - o.z) * rdz;
             if (a > b) {
                 const float tmp = a;
                 a = b;

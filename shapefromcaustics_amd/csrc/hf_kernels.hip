@@ -41,8 +41,10 @@ struct WsLayout {
 WsLayout ws_layout(int32_t R) {
     WsLayout L;
     size_t o = 0;
-    L.hmax = o;  // [0]: max h; [64, 64 + kHmaxBlocks): per-block maxima
-    o = align256(o + sizeof(float) * (64 + kHmaxBlocks));
+    // [0]: max h, [1]: min h; [64, 64 + kHmaxBlocks): per-block maxima, then
+    // kHmaxBlocks per-block minima
+    L.hmax = o;
+    o = align256(o + sizeof(float) * (64 + 2 * kHmaxBlocks));
     const size_t n = (size_t)R * R;
     L.grad_n = o;
     o = align256(o + sizeof(float) * 3 * n);
@@ -81,36 +83,64 @@ __global__ __launch_bounds__(256) void normals_kernel(const float* __restrict__ 
     n[3 * k + 2] = v.z;
 }
 
-// max(h) into ws[0], one workgroup
-// max h in two stages: per-block maxima over a grid of kHmaxBlocks
-// workgroups (one workgroup alone walks 2^16 heights latency-bound), then
-// hmax_kernel over those maxima
+// max and min of h in two stages: per-block extremes over a grid of
+// kHmaxBlocks workgroups (one workgroup alone walks 2^16 heights
+// latency-bound), then hmax_kernel over those.  part: gridDim.x maxima, then
+// gridDim.x minima.
 __global__ __launch_bounds__(256) void hmax_part_kernel(const float* __restrict__ h, int64_t n,
                                                         float* __restrict__ part) {
-    __shared__ float wpart[4];
-    float m = -INFINITY;
-    for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (int64_t)gridDim.x * 256)
+    __shared__ float wpart[4], wmin[4];
+    float m = -INFINITY, mn = INFINITY;
+    for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (int64_t)gridDim.x * 256) {
         m = fmaxf(m, h[k]);
-    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
-    if ((threadIdx.x & 63) == 0) wpart[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) part[blockIdx.x] = fmaxf(fmaxf(wpart[0], wpart[1]), fmaxf(wpart[2], wpart[3]));
-}
-
-// out[0] = the maximum (and copy[0] too, when copy is not null)
-__global__ __launch_bounds__(1024) void hmax_kernel(const float* __restrict__ h, int64_t n, float* __restrict__ out,
-                                                    float* __restrict__ copy) {
-    __shared__ float part[16];
-    float m = -INFINITY;
-    for (int64_t k = threadIdx.x; k < n; k += 1024) m = fmaxf(m, h[k]);
-    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+        mn = fminf(mn, h[k]);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        m = fmaxf(m, __shfl_xor(m, o));
+        mn = fminf(mn, __shfl_xor(mn, o));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        wpart[threadIdx.x >> 6] = m;
+        wmin[threadIdx.x >> 6] = mn;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        float r = -INFINITY;
-        for (int w = 0; w < 16; ++w) r = fmaxf(r, part[w]);
+        part[blockIdx.x] = fmaxf(fmaxf(wpart[0], wpart[1]), fmaxf(wpart[2], wpart[3]));
+        part[gridDim.x + blockIdx.x] = fminf(fminf(wmin[0], wmin[1]), fminf(wmin[2], wmin[3]));
+    }
+}
+
+// parts: n per-block maxima, then n minima.  out[0] = the maximum, out[1] =
+// the minimum (and copy[0..1] too, when copy is not null)
+__global__ __launch_bounds__(1024) void hmax_kernel(const float* __restrict__ parts, int64_t n,
+                                                    float* __restrict__ out, float* __restrict__ copy) {
+    __shared__ float part[16], pmin[16];
+    float m = -INFINITY, mn = INFINITY;
+    for (int64_t k = threadIdx.x; k < n; k += 1024) {
+        m = fmaxf(m, parts[k]);
+        mn = fminf(mn, parts[n + k]);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        m = fmaxf(m, __shfl_xor(m, o));
+        mn = fminf(mn, __shfl_xor(mn, o));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        part[threadIdx.x >> 6] = m;
+        pmin[threadIdx.x >> 6] = mn;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float r = -INFINITY, rn = INFINITY;
+        for (int w = 0; w < 16; ++w) {
+            r = fmaxf(r, part[w]);
+            rn = fminf(rn, pmin[w]);
+        }
         out[0] = r;
-        if (copy) copy[0] = r;
+        out[1] = rn;
+        if (copy) {
+            copy[0] = r;
+            copy[1] = rn;
+        }
     }
 }
 
@@ -122,7 +152,8 @@ SceneArgs scene_args(const float* h, const float* normals, const hf_scene* sc, c
     S.delta = (float)(2.0 / (double)(sc->R - 1));
     S.inv_delta = (float)((double)(sc->R - 1) / 2.0);
     S.sensor_z = sc->sensor_z;
-    S.hmax = 0.0f;  // read from hmax_dev on the device
+    S.hmax = 0.0f;  // hmax and hmin: read from hmax_dev[0..1] on the device
+    S.hmin = 0.0f;
     (void)hmax_dev;
     return S;
 }
@@ -137,6 +168,7 @@ __global__ __launch_bounds__(256) void trace_kernel(SceneArgs S, const float* __
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= N) return;
     S.hmax = hmax[0];
+    S.hmin = hmax[1];
     const V3 o = v3(org[3 * i], org[3 * i + 1], org[3 * i + 2]);
     const V3 d = v3(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]);
     const Hit hit = trace(S, o, d);
@@ -361,6 +393,7 @@ __global__ __launch_bounds__(256, 5) void simulate_kernel(SceneArgs S, const flo
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= O.N) return;
     S.hmax = hmax[0];
+    S.hmin = hmax[1];
     int ch;
     PathState s = emit(E, i, ch);
     const Etas et = etas_of(1.0f, ior_of(E, ch));  // outside: index 1
@@ -672,6 +705,7 @@ __global__ __launch_bounds__(256) void adjoint_kernel(SceneArgs S, const float* 
     __shared__ double pn[kPatch * kPatch * 3];
     __shared__ double pz[kPatch * kPatch];
     S.hmax = hmax[0];
+    S.hmin = hmax[1];
     const int ch = blockIdx.y;
     const int tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
     const int col = tx * 16 + (threadIdx.x & 15), row = ty * 16 + (threadIdx.x >> 4);
@@ -855,8 +889,8 @@ __global__ __launch_bounds__(256) void vcons_kernel(const float* __restrict__ x,
 
 unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
 
-// hmax[0] = max of the n heights (hmax: the workspace's hmax region); also
-// written to copy[0] when copy is not null
+// hmax[0] = max and hmax[1] = min of the n heights (hmax: the workspace's
+// hmax region); also written to copy[0..1] when copy is not null
 void launch_hmax(const float* h, int64_t n, float* hmax, hipStream_t s, float* copy = nullptr) {
     hipLaunchKernelGGL(hmax_part_kernel, dim3(kHmaxBlocks), dim3(256), 0, s, h, n, hmax + 64);
     hipLaunchKernelGGL(hmax_kernel, dim3(1), dim3(1024), 0, s, (const float*)(hmax + 64), (int64_t)kHmaxBlocks, hmax,
@@ -914,8 +948,8 @@ int hip_status(const char* what) {
 }
 
 // hit buffer: records [kHitSlots][N] float4, counts [N] bytes, and the
-// maximum height of the field the records were traced on (the adjoint's
-// re-traced paths use it, so it is not computed again)
+// maximum and minimum height of the field the records were traced on (the
+// adjoint's re-traced paths use them, so they are not computed again)
 struct HitsLayout {
     size_t recs, count, hmax, end;
 };
@@ -925,7 +959,7 @@ HitsLayout hits_layout(int64_t N) {
     L.recs = 0;
     L.count = (sizeof(float4) * kHitSlots * (size_t)N + 255) & ~(size_t)255;
     L.hmax = (L.count + (size_t)N + 255) & ~(size_t)255;
-    L.end = L.hmax + sizeof(float);
+    L.end = L.hmax + 2 * sizeof(float);  // max and min height
     return L;
 }
 

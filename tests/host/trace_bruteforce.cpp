@@ -29,14 +29,15 @@ int main(int argc, char** argv) {
         for (int field = 0; field < 2; ++field) {
             std::vector<float> h(R * R), n(3 * R * R, 0.f);
             const float a1 = 0.05f + 0.1f * U(g), f1 = 2 + 6 * U(g);
-            float hm = 0;
+            float hm = 0, hn = INFINITY;
             for (int r = 0; r < R; ++r)
                 for (int c = 0; c < R; ++c) {
                     const float x = -1 + 2.f * c / (R - 1), y = -1 + 2.f * r / (R - 1);
                     h[r * R + c] = 0.2f + a1 * sinf(f1 * x) * cosf(f1 * y);
                     hm = fmaxf(hm, h[r * R + c]);
+                    hn = fminf(hn, h[r * R + c]);
                 }
-            const SceneArgs S{h.data(), n.data(), R, 2.0f / (R - 1), (R - 1) / 2.0f, -10.f, hm};
+            const SceneArgs S{h.data(), n.data(), R, 2.0f / (R - 1), (R - 1) / 2.0f, -10.f, hm, hn};
             const int nprim = 4 * (R - 1) * (R - 1) + 8 * (R - 1);
             for (int k = 0; k < rays_per_field; ++k) {
                 const float ox = -1.3f + 2.6f * U(g), oy = -1.3f + 2.6f * U(g);
