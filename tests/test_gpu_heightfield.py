@@ -288,3 +288,16 @@ def test_adjoint_replaying_recorded_hits_matches_tracing(case):
     diff = float((g0 - g1).abs().max()) / scale
     print(f"{case}: run-to-run {noise:.2e}, replay vs trace {diff:.2e} (of max |dL/dh|)")
     assert diff <= 1e-5
+
+
+def test_hit_buffer_keeps_the_fields_height_range():
+    # the simulate stores max h and min h after the hit counts; the adjoint
+    # replaying the hits takes the tracer's slab from there
+    R, n_side = 33, 40
+    h = (_h(R, 12, n_bumps=6, amp=(0.05, 0.2), sigma=(0.1, 0.3)) + 0.1).float().cuda()
+    sc, em = HF.Scene(R, -4.0, (1.5,)), HF.ParallelEmitter(n_side, 2.0, 1.0)
+    hits = HF.simulate(h, sc, em, HF.normals(h), keep_hits=True)[5]
+    N = n_side * n_side
+    off = ((16 * 4 * N + 255) // 256 * 256 + N + 255) // 256 * 256
+    stored = hits[off:off + 8].cpu().view(torch.float32)
+    assert float(stored[0]) == float(h.max()) and float(stored[1]) == float(h.min())
