@@ -45,6 +45,7 @@ struct GenArgs {
     int64_t n;
     int32_t W, H, view, n_lines;
     const float* lines;  // [n_lines, 4]: mu_x, mu_y (px), axis angle, sigma_long (px)
+    float inv_n, two_inv_w, two_inv_h;  // 1/n, 2/W, 2/H
     float *E, *xp, *M, *radius;
     int64_t* cp;
     int64_t stride;  // N of the destination arrays (xp [2, stride], M [2, 3, stride])
@@ -70,9 +71,11 @@ __global__ __launch_bounds__(256) void caustic_kernel(GenArgs A) {
         const int k = min((int)(u_k * (float)A.n_lines), A.n_lines - 1);
         const float* L = A.lines + 4 * k;
         const float rad = sqrtf(-2.0f * logf(1.0f - u1));  // 1 - u1 in (0, 1]
-        const float z1 = rad * cosf(6.28318530717958648f * u2), z2 = rad * sinf(6.28318530717958648f * u2);
-        const float a = L[3] * z1, b = (L[3] / 20.0f) * z2;
-        const float c = cosf(L[2]), s = sinf(L[2]);
+        float sb, cb, c, s;
+        sincosf(6.28318530717958648f * u2, &sb, &cb);  // one range reduction for both
+        sincosf(L[2], &s, &c);
+        const float z1 = rad * cb, z2 = rad * sb;
+        const float a = L[3] * z1, b = (L[3] * 0.05f) * z2;
         px = L[0] + c * a - s * b;
         py = L[1] + s * a + c * b;
     } else {
@@ -84,20 +87,24 @@ __global__ __launch_bounds__(256) void caustic_kernel(GenArgs A) {
     const float theta = 3.14159265358979324f * u_th;
     const float s1 = 0.3f * expf(u_s * logf(20.0f));  // logU(0.3, 6)
     const float kappa = 1.0f + 9.0f * u_kap;
-    const float s2 = s1 / kappa;
-    const float c = cosf(theta), s = sinf(theta);
+    float c, s;
+    sincosf(theta, &s, &c);
+    // reciprocals instead of divisions (the kernel is bound by its VALU
+    // instructions): s2 = s1 / kappa, so 1/s2 = kappa/s1 and max(s1, s2) = s1
+    // (kappa >= 1); 1/n, 2/W and 2/H come from the host
+    const float inv_s1 = 1.0f / s1, inv_s2 = kappa * inv_s1;
     const int64_t j = A.offset + i, N = A.stride;
-    A.E[j] = (0.5f + u_e) / (float)A.n;
-    A.xp[j] = 2.0f * px / W - 1.0f;
-    A.xp[N + j] = 2.0f * py / H - 1.0f;
+    A.E[j] = (0.5f + u_e) * A.inv_n;
+    A.xp[j] = px * A.two_inv_w - 1.0f;
+    A.xp[N + j] = py * A.two_inv_h - 1.0f;
     // D^-1 = diag(1/s1, 1/s2) R^T, times diag(W/2, H/2)
-    A.M[j] = c / s1 * (0.5f * W);
-    A.M[N + j] = s / s1 * (0.5f * H);
+    A.M[j] = c * inv_s1 * (0.5f * W);
+    A.M[N + j] = s * inv_s1 * (0.5f * H);
     A.M[2 * N + j] = 0.0f;
-    A.M[3 * N + j] = -s / s2 * (0.5f * W);
-    A.M[4 * N + j] = c / s2 * (0.5f * H);
+    A.M[3 * N + j] = -s * inv_s2 * (0.5f * W);
+    A.M[4 * N + j] = c * inv_s2 * (0.5f * H);
     A.M[5 * N + j] = 0.0f;
-    A.radius[j] = 2.0f * fmaxf(s1, s2) / W;
+    A.radius[j] = s1 * A.two_inv_w;
     A.cp[j] = 3 * (int64_t)A.view + (i % 3);
 }
 
@@ -133,6 +140,9 @@ int pds_gen_caustic_f32(uint64_t seed, int32_t view, int64_t n, int32_t W, int32
     A.view = view;
     A.n_lines = n_lines;
     A.lines = lines;
+    A.inv_n = (float)(1.0 / (double)n);
+    A.two_inv_w = (float)(2.0 / (double)W);
+    A.two_inv_h = (float)(2.0 / (double)H);
     A.E = Ep;
     A.xp = xp;
     A.M = Mp;

@@ -248,6 +248,21 @@ def test_deterministic_outside_binned_range(size, cap):
     assert torch.equal(a, b)
 
 
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("cap", [0, -1])
+def test_degenerate_window_caps_and_negative_radii(mode, cap):
+    # cap 0: every window is the centre pixel; cap -1: every window is empty
+    # (zero image, zero gradients); negated radii: ceil(radius W / 2) <= 0
+    p = WL.plain(3000, (2, 64, 64), cap, seed=5)
+    p.radius[:50] = -p.radius[:50]
+    g = WL.upstream_grad(p.size, seed=5)
+    img, grads = _run(p, mode, g)
+    assert_image_close(img, O.forward_f64(*p.arrays(), p.size, cap), f"cap {cap} fwd")
+    assert_grads_close(grads, O.backward_f64(g, *p.arrays(), cap), f"cap {cap} bwd")
+    if cap < 0:
+        assert not img.any() and not any(t.any() for t in grads)
+
+
 @pytest.mark.parametrize("mode", ["auto", "binned"])
 def test_cap_above_binned_range_uses_photon_order_kernels(mode):
     # cap 70 > 63 on a 256^2 image: the window ellipse test stays inactive
