@@ -42,7 +42,10 @@ extern "C" {
 /* Height field h [R, R] (row = y, column = x) over [-1, 1]^2, vertex (r, c)
  * at (-1 + 2c/(R-1), -1 + 2r/(R-1), h[r, c]); flat bottom z = 0; ground plane
  * z = sensor_z over [-1, 1]^2.  Photon i has channel i mod channels and sees
- * the index ior[channel] inside the glass (1 outside). */
+ * the index ior[channel] inside the glass (1 outside).  Limits: 2 <= R <=
+ * 16384, 1 <= channels <= HF_MAX_CHANNELS, 1 <= n_side <= 32768 and
+ * N = n_side^2 * channels <= 2^31 - 1 (else PDS_ERR_ARGUMENT /
+ * PDS_ERR_CAPACITY, and the *_bytes queries return 0). */
 typedef struct hf_scene {
     int32_t R;
     int32_t channels;

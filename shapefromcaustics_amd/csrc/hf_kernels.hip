@@ -923,9 +923,16 @@ EmitArgs emit_args(const hf_scene* sc, const hf_emitter* em) {
     return E;
 }
 
+// Size limits: 3 R^2 normal components and N = n_side^2 channels photons
+// stay below 2^31, so 32-bit vertex indices and one-dimensional launches of
+// one thread per photon hold.
+constexpr int32_t kMaxR = 16384;
+constexpr int32_t kMaxSide = 32768;
+constexpr int64_t kMaxPhotons = ((int64_t)1 << 31) - 1;
+
 int check_scene(const hf_scene* sc, const void* h, const void* normals) {
     if (!sc) return fail(PDS_ERR_ARGUMENT, "null scene");
-    if (sc->R < 2 || sc->R > 32768) return fail(PDS_ERR_ARGUMENT, "scene R must be in [2, 32768]");
+    if (sc->R < 2 || sc->R > kMaxR) return fail(PDS_ERR_ARGUMENT, "scene R must be in [2, 16384]");
     if (sc->channels < 1 || sc->channels > HF_MAX_CHANNELS) return fail(PDS_ERR_ARGUMENT, "channels must be in [1, 8]");
     if (!h || !normals) return fail(PDS_ERR_ARGUMENT, "null height field or normals");
     return PDS_OK;
@@ -933,7 +940,7 @@ int check_scene(const hf_scene* sc, const void* h, const void* normals) {
 
 int check_emitter(const hf_emitter* em) {
     if (!em) return fail(PDS_ERR_ARGUMENT, "null emitter");
-    if (em->n_side < 1 || em->n_side > 65536) return fail(PDS_ERR_ARGUMENT, "emitter n_side must be in [1, 65536]");
+    if (em->n_side < 1 || em->n_side > kMaxSide) return fail(PDS_ERR_ARGUMENT, "emitter n_side must be in [1, 32768]");
     if (em->kind != 0 && em->kind != 1) return fail(PDS_ERR_ARGUMENT, "emitter kind must be 0 (parallel) or 1 (point)");
     return PDS_OK;
 }
@@ -974,6 +981,7 @@ int simulate_impl(const float* h, const float* normals, const hf_scene* sc, cons
     const WsLayout L = ws_layout(sc->R);
     if (!ws || ws_bytes < L.end) return fail(PDS_ERR_WORKSPACE, "workspace smaller than hf_workspace_bytes(R)");
     const int64_t N = (int64_t)em->n_side * em->n_side * sc->channels;
+    if (N > kMaxPhotons) return fail(PDS_ERR_CAPACITY, "n_side^2 * channels photons exceed 2^31 - 1");
     const HitsLayout HL = hits_layout(N);
     if (want_hits && (!hits || hits_bytes < HL.end))
         return fail(PDS_ERR_WORKSPACE, "hit buffer smaller than hf_hits_bytes");
@@ -999,6 +1007,7 @@ int adjoint_impl(const float* h, const float* normals, const hf_scene* sc, const
     const WsLayout L = ws_layout(sc->R);
     if (!ws || ws_bytes < L.end) return fail(PDS_ERR_WORKSPACE, "workspace smaller than hf_workspace_bytes(R)");
     const int64_t N = (int64_t)em->n_side * em->n_side * sc->channels;
+    if (N > kMaxPhotons) return fail(PDS_ERR_CAPACITY, "n_side^2 * channels photons exceed 2^31 - 1");
     const HitsLayout HL = hits_layout(N);
     if (want_hits && (!hits || hits_bytes < HL.end))
         return fail(PDS_ERR_WORKSPACE, "hit buffer smaller than hf_hits_bytes");
@@ -1042,11 +1051,11 @@ extern "C" {
 
 const char* hf_last_error(void) { return g_err; }
 
-size_t hf_workspace_bytes(int32_t R) { return R < 2 ? 0 : ws_layout(R).end; }
+size_t hf_workspace_bytes(int32_t R) { return R < 2 || R > kMaxR ? 0 : ws_layout(R).end; }
 
 int hf_normals_f32(const float* h, int32_t R, float* normals, void* stream) {
     g_err[0] = '\0';
-    if (R < 2 || !h || !normals) return fail(PDS_ERR_ARGUMENT, "invalid height field");
+    if (R < 2 || R > kMaxR || !h || !normals) return fail(PDS_ERR_ARGUMENT, "invalid height field (R must be in [2, 16384])");
     const float s = (float)(-1.0 / (8.0 * (2.0 / (double)(R - 1))));
     hipLaunchKernelGGL(normals_kernel, dim3(blocks_for((int64_t)R * R)), dim3(256), 0, (hipStream_t)stream, h, R, s,
                        normals);
@@ -1073,9 +1082,10 @@ int hf_trace_f32(const float* h, const float* normals, const hf_scene* sc, const
 }
 
 size_t hf_hits_bytes(const hf_scene* sc, const hf_emitter* em) {
-    if (!sc || !em || sc->channels < 1 || em->n_side < 1) return 0;
+    if (!sc || !em || sc->channels < 1 || sc->channels > HF_MAX_CHANNELS || em->n_side < 1 || em->n_side > kMaxSide)
+        return 0;
     const int64_t N = (int64_t)em->n_side * em->n_side * sc->channels;
-    return hits_layout(N).end;
+    return N > kMaxPhotons ? 0 : hits_layout(N).end;
 }
 
 int hf_simulate_f32(const float* h, const float* normals, const hf_scene* sc, const hf_emitter* em, float* Ep,
@@ -1118,13 +1128,13 @@ int hf_residual_f32(const float* img, const float* target, int64_t n, float* res
 }
 
 size_t hf_landweber_workspace_bytes(int32_t R) {
-    return R < 1 ? 0 : align256(sizeof(float) * kRedBlocks) + sizeof(float) * (size_t)R * R;
+    return R < 1 || R > kMaxR ? 0 : align256(sizeof(float) * kRedBlocks) + sizeof(float) * (size_t)R * R;
 }
 
 int hf_landweber_f32(float* d, const float* grad, const float* mask, int32_t R, const hf_landweber* p, void* ws,
                      size_t ws_bytes, void* stream) {
     g_err[0] = '\0';
-    if (R < 1 || !d || !grad || !mask || !p) return fail(PDS_ERR_ARGUMENT, "invalid Landweber args");
+    if (R < 1 || R > kMaxR || !d || !grad || !mask || !p) return fail(PDS_ERR_ARGUMENT, "invalid Landweber args");
     if (p->volume_radius < 0 || p->volume_radius > 64) return fail(PDS_ERR_ARGUMENT, "volume_radius must be in [0, 64]");
     if (!ws || ws_bytes < hf_landweber_workspace_bytes(R)) return fail(PDS_ERR_WORKSPACE, "Landweber workspace too small");
     hipStream_t s = (hipStream_t)stream;

@@ -56,6 +56,30 @@ def test_heightfield_and_generator_argument_errors():
     assert hl.hf_workspace_bytes(1) == 0 and hl.hf_workspace_bytes(16) > 16 * 16 * 4
 
 
+def test_heightfield_size_limits():
+    # 3 R^2 normal components and n_side^2 * channels photons stay below 2^31
+    from shapefromcaustics_amd import heightfield
+    hl = heightfield.lib()
+    h = ctypes.c_void_p(1)
+    assert hl.hf_workspace_bytes(16384) > 0 and hl.hf_workspace_bytes(16385) == 0
+    assert hl.hf_landweber_workspace_bytes(16385) == 0
+    rc = hl.hf_normals_f32(h, 16385, h, None)
+    assert rc == _native.PDS_ERR_ARGUMENT and b"16384" in hl.hf_last_error()
+    sc = heightfield.Scene(16385, -1.0).struct()
+    rc = hl.hf_simulate_f32(h, h, ctypes.byref(sc), None, None, None, None, None, None, None, 0, None)
+    assert rc == _native.PDS_ERR_ARGUMENT and b"R must be" in hl.hf_last_error()
+    sc = heightfield.Scene(8, -1.0, iors=(1.5, 1.5)).struct()
+    em = heightfield.ParallelEmitter(32769, 1.0).struct()
+    assert hl.hf_hits_bytes(ctypes.byref(sc), ctypes.byref(em)) == 0
+    rc = hl.hf_simulate_f32(h, h, ctypes.byref(sc), ctypes.byref(em), h, h, h, h, h, h, 1 << 20, None)
+    assert rc == _native.PDS_ERR_ARGUMENT and b"n_side" in hl.hf_last_error()
+    em = heightfield.ParallelEmitter(32768, 1.0).struct()             # 2 channels: 2^31 photons
+    assert hl.hf_hits_bytes(ctypes.byref(sc), ctypes.byref(em)) == 0
+    ws = hl.hf_workspace_bytes(8)
+    rc = hl.hf_adjoint_f32(h, h, ctypes.byref(sc), ctypes.byref(em), h, h, h, h, h, ws, None)
+    assert rc == _native.PDS_ERR_CAPACITY and b"2^31" in hl.hf_last_error()
+
+
 def test_version_and_empty_error():
     assert b"gfx950" in _native.lib().pds_version()
 
