@@ -199,9 +199,11 @@ struct RawPhoton {
     float e, cx, cy, radius, m00, m01, m02, m10, m11, m12;
 };
 
-// with_e = false: the energy is not read (0), for passes that only need the
-// photon's footprint (which does not depend on it)
-__device__ __forceinline__ RawPhoton load_photon(const PhotonArgs& P, int64_t i, bool with_e = true) {
+// with_e = false: the energy is not read (0); with_m2 = false: nor is M[:, 2]
+// (0) -- for passes that only need the photon's footprint, which depends on
+// neither
+__device__ __forceinline__ RawPhoton load_photon(const PhotonArgs& P, int64_t i, bool with_e = true,
+                                                 bool with_m2 = true) {
     const int64_t N = P.N;
     RawPhoton r;
     r.c = P.cp[i];
@@ -211,10 +213,10 @@ __device__ __forceinline__ RawPhoton load_photon(const PhotonArgs& P, int64_t i,
     r.radius = P.radius[i];
     r.m00 = P.M[i];
     r.m01 = P.M[N + i];
-    r.m02 = P.M[2 * N + i];
+    r.m02 = with_m2 ? P.M[2 * N + i] : 0.0f;
     r.m10 = P.M[3 * N + i];
     r.m11 = P.M[4 * N + i];
-    r.m12 = P.M[5 * N + i];
+    r.m12 = with_m2 ? P.M[5 * N + i] : 0.0f;
     return r;
 }
 
@@ -234,10 +236,13 @@ __device__ __forceinline__ BinPhoton bin_photon(const RawPhoton& R, const ImageA
     int32_t y1 = (int32_t)min((int64_t)win.pyc + win.ry, (int64_t)I.H - 1);
     if (x0 > x1 || y0 > y1) return b;
     const float m00 = R.m00, m01 = R.m01, m10 = R.m10, m11 = R.m11;
+    // 0, or NaN when M[:, 2] is not finite: such a photon keeps its records
+    // (so the count pass need not read M[:, 2]) with a NaN energy, which the
+    // forward's val > 0 test drops, as the reference's NaN q does
     const float poison = R.m02 * 0.0f + R.m12 * 0.0f;
     const bool finite_frame = isfinite(cx) && isfinite(cy) && isfinite(m00) && isfinite(m01) && isfinite(m10) &&
-                              isfinite(m11) && poison == 0.0f;
-    b.special = !finite_frame || !isfinite(e);
+                              isfinite(m11);
+    b.special = !finite_frame || !isfinite(e) || poison != 0.0f;
     if (!finite_frame) return b;
     b.f = photon_frame(cx, cy, m00, m01, m10, m11, 0.0f, I.W, I.H, I.inv_hw, I.inv_hh);
     clip_to_ellipse(b.f, x0, x1, y0, y1);
@@ -247,7 +252,7 @@ __device__ __forceinline__ BinPhoton bin_photon(const RawPhoton& R, const ImageA
     b.y0 = y0;
     b.y1 = y1;
     b.c = (int32_t)c;
-    b.e = e;
+    b.e = e + poison;
     b.rx = win.rx;
     b.ry = win.ry;
     b.ox = win.pxc - b.f.ix + 1;  // in {0, 1, 2}: trunc vs round of the same double
@@ -303,12 +308,12 @@ __global__ __launch_bounds__(kBinThreads) void bin_count_kernel(PhotonArgs P, Im
     int64_t i = lo + threadIdx.x;
     // two photons of this lane in flight while one is binned
     RawPhoton r0 = {}, r1 = {};
-    if (i < hi) r0 = load_photon(P, i, g.det);
-    if (i + kBinThreads < hi) r1 = load_photon(P, i + kBinThreads, g.det);
+    if (i < hi) r0 = load_photon(P, i, g.det, false);
+    if (i + kBinThreads < hi) r1 = load_photon(P, i + kBinThreads, g.det, false);
     for (; i < hi; i += kBinThreads) {
         const RawPhoton now = r0;
         r0 = r1;
-        if (i + 2 * kBinThreads < hi) r1 = load_photon(P, i + 2 * kBinThreads, g.det);
+        if (i + 2 * kBinThreads < hi) r1 = load_photon(P, i + 2 * kBinThreads, g.det, false);
         const BinPhoton b = bin_photon(now, I);
         if (!b.rec) continue;
         if (b.e > 0.0f && isfinite(b.e)) emax = fmaxf(emax, b.e);
