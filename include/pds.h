@@ -62,8 +62,8 @@ typedef enum pds_mode {
 typedef struct pds_tuning {
     int32_t tile;          /* square screen tile edge in pixels: 8, 16, 32 or 64 */
     int32_t seg_photons;   /* photon records per tile segment (one workgroup each), <= 2048 */
-    int32_t block_photons; /* photons per binning workgroup (default 4096; for fewer than
-                              2^20 photons N / 256, but at least 1024) */
+    int32_t block_photons; /* photons per binning workgroup (default 8192; for fewer than
+                              2^21 photons N / 256, but at least 1024) */
     int32_t capacity_pct;  /* records reserved per 100 photons (default 200); more
                               records than that run the photon-order path */
     int32_t walk;          /* 0: walk each footprint's whole bounding box (default);

@@ -48,7 +48,7 @@ constexpr int kTileThreads = 512;
 constexpr int kMaxSeg = 2048;  // records per segment (sizes the LDS sort of a segment)
 constexpr int kSortKeys = 256;  // walk lengths 0..255 (clamped), sorted by one LDS counting sort
 constexpr int kMinBins = 256;
-constexpr int64_t kDefaultBlockPhotons = 4096;     // photons per binning workgroup
+constexpr int64_t kDefaultBlockPhotons = 8192;     // photons per binning workgroup
 constexpr int64_t kMinDefaultBlockPhotons = 1024;  // ... at least, for small problems
 constexpr int64_t kTargetBinBlocks = 256;          // binning workgroups a small problem still gets (one per CU)
 constexpr int kDefaultSeg = 2048;
@@ -115,7 +115,7 @@ bool make_geom(int64_t N, int32_t C, int32_t H, int32_t W, int32_t cap, int32_t 
     g.nbins = C * g.tiles_x * g.tiles_y;
     g.seg = (t && t->seg_photons > 0) ? t->seg_photons : kDefaultSeg;
     if (g.seg > kMaxSeg) g.seg = kMaxSeg;
-    // default photons per binning block: 4096, fewer for small problems so that
+    // default photons per binning block: 8192, fewer for small problems so that
     // count and scatter still launch ~256 workgroups, but at least 1024
     const int64_t bp_default = std::min(kDefaultBlockPhotons, std::max(kMinDefaultBlockPhotons,
                                                                         (N + kTargetBinBlocks - 1) / kTargetBinBlocks));

@@ -278,9 +278,9 @@ def test_cap_above_binned_range_uses_photon_order_kernels(mode):
 
 
 @pytest.mark.parametrize("n,want", [(1 << 12, 1024), (1 << 18, 1024), (1 << 19, 2048), (1 << 20, 4096),
-                                    (1 << 22, 4096)])
+                                    (1 << 21, 8192), (1 << 22, 8192)])
 def test_default_photons_per_binning_block(n, want):
-    # 4096, or N / 256 (at least 1024) for small problems so that ~256 binning workgroups still run
+    # 8192, or N / 256 (at least 1024) for small problems so that ~256 binning workgroups still run
     from shapefromcaustics_amd.PhotonDifferentialSplatting import _bins_stats, _forward
     p = WL.plain(n, (1, 256, 256), 8, seed=4)
     E, xp, M, cp, radius = _dev(p)
