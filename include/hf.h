@@ -20,9 +20,11 @@
  * geometric normals (vertex heights) and the differentials, not through the
  * hit positions' dependence on h.  The reference's dN (renderable_object.py
  * :391-401) projects (sum_v L_v . x) n_s orthogonally to n_s and is
- * identically zero; it is zero here.  Bounces are capped at 8 (the reference
- * loops until every photon has left the scene); rays still bouncing after 8
- * are dropped.
+ * identically zero; it is zero here.  Paths are capped at HF_MAX_BOUNCES
+ * hits (the reference loops until every photon has left the scene,
+ * model/caustics.py:255; a GPU thread needs a bound): a photon still
+ * bouncing after HF_MAX_BOUNCES hits is dropped, and the hit-recording
+ * simulate counts such photons (hf_hits_stats_get).
  *
  * Conventions as in pds.h: device pointers, caller-owned buffers and
  * workspace, a hipStream_t passed as void*, 0 or a negative pds_status.
@@ -38,6 +40,7 @@ extern "C" {
 #endif
 
 #define HF_MAX_CHANNELS 8
+#define HF_MAX_BOUNCES 16
 
 /* Height field h [R, R] (row = y, column = x) over [-1, 1]^2, vertex (r, c)
  * at (-1 + 2c/(R-1), -1 + 2r/(R-1), h[r, c]); flat bottom z = 0; ground plane
@@ -100,7 +103,8 @@ int hf_trace_f32(const float* h, const float* normals, const hf_scene* scene, co
  * write its splat inputs (model/caustics.py:233-239) in the pds.h layout:
  * Ep [N], xp [2, N], Mp [2, 3, N], cp [1, N], radius [N], N = n_side^2 *
  * channels.  Photons that miss, are dropped by the reference's masks, or
- * bounce more than 8 times get cp = -1 (pds_* skip them). */
+ * are still bouncing after HF_MAX_BOUNCES hits get cp = -1 (pds_* skip
+ * them). */
 int hf_simulate_f32(const float* h, const float* normals, const hf_scene* scene, const hf_emitter* emitter,
                     float* Ep, float* xp, float* Mp, int64_t* cp, float* radius, void* workspace,
                     size_t workspace_bytes, void* stream);
@@ -124,6 +128,20 @@ int hf_adjoint_f32(const float* h, const float* normals, const hf_scene* scene, 
  * hf_adjoint_f32 bit for bit; dL/dh differs only by the (run-dependent)
  * order of the float atomic sums. */
 size_t hf_hits_bytes(const hf_scene* scene, const hf_emitter* emitter);
+
+/* How the paths of the hf_simulate_hits_f32 that wrote `hits` ended: read
+ * from the buffer (synchronises `stream`; a diagnostic, not for the hot
+ * loop).  reached + missed + degenerate + capped = N.  No reference
+ * counterpart (the reference drops such photons silently). */
+typedef struct hf_path_stats {
+    int64_t reached;     /* reached the sensor with a valid splat (cp >= 0) */
+    int64_t missed;      /* left the scene: a trace found no hit */
+    int64_t degenerate;  /* reached the sensor, dropped by the reference's den != 0 mask */
+    int64_t capped;      /* still bouncing after HF_MAX_BOUNCES hits: dropped */
+    int64_t reserved[4];
+} hf_path_stats;
+int hf_hits_stats_get(const void* hits, size_t hits_bytes, const hf_scene* scene, const hf_emitter* emitter,
+                      hf_path_stats* out, void* stream);
 int hf_simulate_hits_f32(const float* h, const float* normals, const hf_scene* scene, const hf_emitter* emitter,
                          float* Ep, float* xp, float* Mp, int64_t* cp, float* radius, void* workspace,
                          size_t workspace_bytes, void* hits, size_t hits_bytes, void* stream);

@@ -65,7 +65,9 @@ typedef struct pds_tuning {
     int32_t block_photons; /* photons per binning workgroup (default 8192; for fewer than
                               2^21 photons N / 256, but at least 1024) */
     int32_t capacity_pct;  /* records reserved per 100 photons (default 200); more
-                              records than that run the photon-order path */
+                              records than that run the photon-order path (so do
+                              capacities of 2^31 records or more, which the
+                              binned path's 32-bit record offsets cannot hold) */
     int32_t walk;          /* 0: walk each footprint's whole bounding box (default);
                               1: walk only the ellipse's row chords (same results,
                               bit for bit: the pixels left out contribute exactly
@@ -143,6 +145,27 @@ int pds_backward_bins_f32(const float* grad, const float* Ep, const float* xp, c
                           int32_t W, int32_t max_pixel_radius, int32_t mode, const pds_tuning* t,
                           float* grad_Ep, float* grad_xp, float* grad_Mp, void* bins, size_t bins_bytes,
                           void* stream);
+
+/* ---- batches with a device-side photon count ---------------------------
+ * No counterpart in the reference (it splats its num_simulations inner
+ * simulations one after another, shape_from_caustics.py:351-353).  The
+ * bins-sharing pair for a batch buffer of capacity N whose photons beyond a
+ * count held in device memory are unused: only photons i < *live_count
+ * (clamped to [0, N]; read by the kernels when they run, so one captured HIP
+ * graph serves every count) are splatted and get gradients, and the
+ * gradients of photons i >= *live_count are not written.  N sizes the bins
+ * (pds_bins_bytes(N, ...)); the work follows the live count.  The backward
+ * takes the same live_count as its forward. */
+int pds_forward_bins_live_f32(const float* Ep, const float* xp, const float* Mp, const int64_t* cp,
+                              const float* radius, int64_t N, int32_t C, int32_t H, int32_t W,
+                              int32_t max_pixel_radius, int32_t mode, const pds_tuning* t,
+                              const int64_t* live_count, float* out, void* bins, size_t bins_bytes,
+                              void* stream);
+int pds_backward_bins_live_f32(const float* grad, const float* Ep, const float* xp, const float* Mp,
+                               const int64_t* cp, const float* radius, int64_t N, int32_t C, int32_t H,
+                               int32_t W, int32_t max_pixel_radius, int32_t mode, const pds_tuning* t,
+                               const int64_t* live_count, float* grad_Ep, float* grad_xp, float* grad_Mp,
+                               void* bins, size_t bins_bytes, void* stream);
 
 /* Message of the calling thread's last failed call ("" if none). */
 const char* pds_last_error(void);

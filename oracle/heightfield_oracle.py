@@ -211,8 +211,10 @@ def differential_normal_literal(V, ns, x):
     return (ndn * dn - dot(ns, dn, keepdim=True) * ns) / ndn ** 1.5
 
 
-def refract_reflect(d, N0, Dth, Dph, dNdt, dNdp, eta_out, eta_in):
-    """model/caustics.py:152-212, restated literally (cosi kept unflipped)."""
+def refract_reflect(d, N0, Dth, Dph, dNdt, dNdp, eta_out, eta_in, return_tir=False):
+    """model/caustics.py:152-212, restated literally (cosi kept unflipped).
+    return_tir: also return the mask of rays that took the total-internal-
+    reflection branch (:205-210)."""
     cosi = dot(d, N0, keepdim=True).clamp(-1, 1)
     flip = cosi > 0
     etai = torch.where(flip, eta_in, eta_out)
@@ -233,7 +235,8 @@ def refract_reflect(d, N0, Dth, Dph, dNdt, dNdp, eta_out, eta_in):
     d_t = d - 2 * cosi * N
     Dt_t = Dth - 2 * (cosi * dNdt + (dot(Dth, N, keepdim=True) + dot(d, dNdt, keepdim=True)) * N)
     Dp_t = Dph - 2 * (cosi * dNdp + (dot(Dph, N, keepdim=True) + dot(d, dNdp, keepdim=True)) * N)
-    return torch.where(rm, d_r, d_t), torch.where(rm, Dt_r, Dt_t), torch.where(rm, Dp_r, Dp_t)
+    out = torch.where(rm, d_r, d_t), torch.where(rm, Dt_r, Dt_t), torch.where(rm, Dp_r, Dp_t)
+    return (*out, ~rm[:, 0]) if return_tir else out
 
 
 def splat_inputs(p, Du, Dv, ns, flux):
@@ -302,6 +305,25 @@ def frisvad_basis(n, eps=1e-5):
     return b1, b2
 
 
+def point_light_dirs(a, b, light, plane_z) -> torch.Tensor:
+    """Directions [n, 3] of the point light (model/caustics.py:38-44): the
+    bilinear slerp, with coefficients a, b [n], of the unit directions from
+    ``light`` to the corners of the square [-1, 1]^2 at height ``plane_z``
+    (lower edge y = -1 and upper edge y = +1 interpolated by a, then the two
+    by b)."""
+    L = torch.tensor(light, dtype=F64)
+    a = torch.as_tensor(a, dtype=F64).reshape(-1, 1)
+    b = torch.as_tensor(b, dtype=F64).reshape(-1, 1)
+    n = a.shape[0]
+
+    def corner(x, y):
+        return normalize(torch.tensor([x, y, plane_z], dtype=F64) - L)[None].expand(n, 3)
+
+    lower = slerp(corner(-1, -1), corner(1, -1), a)
+    upper = slerp(corner(-1, 1), corner(1, 1), a)
+    return slerp(lower, upper, b)
+
+
 def point_light_rays(n_side, channels, light, plane_z, diff_scale, flux, seed) -> Rays:
     """model/caustics.py:26-70 with the interpolation coefficients (a, b) of
     sample s drawn as SplitMix64(seed * 0x100000001B3 ^ (s << 2 | k)), k = 0, 1
@@ -312,16 +334,10 @@ def point_light_rays(n_side, channels, light, plane_z, diff_scale, flux, seed) -
     with np.errstate(over="ignore"):
         key = (np.uint64(seed) * np.uint64(0x100000001B3)) & GT.M64
     ctr = np.arange(n_s, dtype=np.uint64) << np.uint64(2)
-    a = torch.from_numpy(GT.u24(GT.splitmix64(key ^ (ctr | np.uint64(0)))).astype(np.float64))[:, None]
-    b = torch.from_numpy(GT.u24(GT.splitmix64(key ^ (ctr | np.uint64(1)))).astype(np.float64))[:, None]
+    a = torch.from_numpy(GT.u24(GT.splitmix64(key ^ (ctr | np.uint64(0)))).astype(np.float64))
+    b = torch.from_numpy(GT.u24(GT.splitmix64(key ^ (ctr | np.uint64(1)))).astype(np.float64))
     L = torch.tensor(light, dtype=F64)
-
-    def corner(x, y):
-        return normalize(torch.tensor([x, y, plane_z], dtype=F64) - L)[None].expand(n_s, 3)
-
-    lower = slerp(corner(-1, -1), corner(1, -1), a)
-    upper = slerp(corner(-1, 1), corner(1, 1), a)
-    d = slerp(lower, upper, b)
+    d = point_light_dirs(a, b, light, plane_z)
     b1, b2 = frisvad_basis(d)
     rep = lambda t: t.repeat_interleave(channels, 0)
     n = n_s * channels
@@ -342,10 +358,20 @@ class Path:
     cp: torch.Tensor        # [1, n]
     radius: torch.Tensor
     hits: list              # per bounce: (ray index, t, b, tri) -- the detached trace results
+    capped: torch.Tensor = None   # rays still bouncing after max_bounces hits (dropped)
+    tir: torch.Tensor = None      # rays that were totally internally reflected at least once
 
 
-def simulate(h: torch.Tensor, rays: Rays, ior: float, sensor_z: float, max_bounces=16, frozen=None) -> Path:
+MAX_BOUNCES = 16  # the GPU path's cap (HF_MAX_BOUNCES of include/hf.h)
+
+
+def simulate(h: torch.Tensor, rays: Rays, ior: float, sensor_z: float, max_bounces=MAX_BOUNCES,
+             frozen=None) -> Path:
     """model/caustics.py:242-291 for one substrate and a ground plane (object 1).
+
+    The reference loops until every photon has left (:255); this loop stops
+    after ``max_bounces`` hits and drops the rays still bouncing (``capped``),
+    as the GPU path does with the same bound.
 
     ``frozen``: the ``hits`` of an earlier run; the trace is replaced by those
     hits so that the result is the smooth function of h that the reference
@@ -360,6 +386,7 @@ def simulate(h: torch.Tensor, rays: Rays, ior: float, sensor_z: float, max_bounc
     ei = torch.full((n, 1), float(ior), dtype=F64)
     ground = []
     hits = []
+    tir_rays = []
     for bounce in range(max_bounces):
         if ray.numel() == 0:
             break
@@ -388,15 +415,21 @@ def simulate(h: torch.Tensor, rays: Rays, ior: float, sensor_z: float, max_bounc
             a[r] for a in (ray, p, d, Du, Dv, Dth, Dph, ns, V, Dth_hit, Dph_hit, flux, ch, eo, ei))
         dNdt = differential_normal_literal(V, ns, Dth_hit)
         dNdp = differential_normal_literal(V, ns, Dph_hit)
-        d, Dth, Dph = refract_reflect(d, ns, Dth, Dph, dNdt, dNdp, eo, ei)
+        d, Dth, Dph, tir = refract_reflect(d, ns, Dth, Dph, dNdt, dNdp, eo, ei, return_tir=True)
+        tir_rays.append(ray[tir])
         o = p
+    capped = ray.clone()  # still in flight after the last allowed hit
+    tir_mask = torch.zeros(n, dtype=torch.bool)
+    for r in tir_rays:
+        tir_mask[r] = True
     if not ground:
         z = torch.zeros(0, dtype=F64)
         return Path(torch.zeros(0, dtype=torch.int64), z, z.reshape(2, 0), z.reshape(2, 3, 0),
-                    torch.zeros(1, 0, dtype=torch.int64), z, hits)
+                    torch.zeros(1, 0, dtype=torch.int64), z, hits, capped, tir_mask)
     ray, p, Du, Dv, ns, flux, ch = (torch.cat([g[k] for g in ground]) for k in range(7))
     Ep, xp, Mp, radius, good = splat_inputs(p, Du, Dv, ns, flux)
-    return Path(ray[good], Ep[good], xp[:, good], Mp[:, :, good], ch[good][None], radius[good], hits)
+    return Path(ray[good], Ep[good], xp[:, good], Mp[:, :, good], ch[good][None], radius[good], hits, capped,
+                tir_mask)
 
 
 # ----------------------------------------------------------------------------- PDS in float64 (autograd)

@@ -19,6 +19,23 @@
 
 static inline float silverman_f(float q) { return q < 1.0f ? (3.0f / (float)PDS_PI) * (1.0f - q) * (1.0f - q) : 0.0f; }
 
+/* Per-thread private images of the forward, kept between calls (grown when
+ * a larger problem comes) so that no call pays for allocating and faulting
+ * in C x H x W floats per thread.  The baseline is called from one host
+ * thread at a time. */
+static float* g_priv = NULL;
+static size_t g_priv_n = 0;
+
+static float* private_images(size_t n)
+{
+    if (n > g_priv_n) {
+        free(g_priv);
+        g_priv = (float*)malloc(n * sizeof(float));
+        g_priv_n = g_priv ? n : 0;
+    }
+    return g_priv;
+}
+
 int pds_cpu_threads(void)
 {
 #ifdef _OPENMP
@@ -36,7 +53,7 @@ int pds_cpu_forward_f32(const float* E, const float* xp, const float* M, const i
     size_t img = (size_t)C * H * W;
     int nt = nthreads > 0 ? nthreads : pds_cpu_threads();
     const int ell_all = window_ellipse_ok(cap, cap, W, H);
-    float* priv = (float*)calloc(img * (size_t)nt, sizeof(float));
+    float* priv = private_images(img * (size_t)nt);
     if (!priv) return -2;
     const float fw = (float)W, fh = (float)H;
 #pragma omp parallel num_threads(nt)
@@ -47,6 +64,7 @@ int pds_cpu_forward_f32(const float* E, const float* xp, const float* M, const i
         int tid = 0;
 #endif
         float* mine = priv + img * (size_t)tid;
+        memset(mine, 0, img * sizeof(float));  /* each thread clears its own image (first touch) */
 #pragma omp for schedule(static)
         for (int64_t i = 0; i < N; ++i) {
             int64_t c = cp[i];
@@ -80,7 +98,6 @@ int pds_cpu_forward_f32(const float* E, const float* xp, const float* M, const i
             out[p] = s;
         }
     }
-    free(priv);
     return 0;
 }
 

@@ -67,7 +67,7 @@ def _lib(name: str):
                                                        ctypes.c_int32, ctypes.c_int64, ctypes.c_double,
                                                        ctypes.c_double, ctypes.c_double, _f64p, ctypes.c_double]
             lib.pds_oracle_photon_loss_f64.restype = ctypes.c_double
-        else:
+        else:  # libpds_cpu.so, libpds_cpu_native.so
             lib.pds_cpu_forward_f32.argtypes = [_f32p, _f32p, _f32p, _i64p, _f32p, ctypes.c_int64,
                                                 ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                                 ctypes.c_int32, _f32p, ctypes.c_int32]
@@ -153,6 +153,31 @@ def grads_within_tolerance(got, ref: OracleGrads, rtol=1e-3):
     return worst <= 1.0, worst
 
 
+def grad_error_report(got, ref: OracleGrads, rtol=1e-3) -> dict:
+    """How the gradients ``got`` compare with the oracle under the plain
+    criterion |g - g*| <= rtol * sum|terms| (without the FP32 conditioning
+    bound that :func:`grads_within_tolerance` adds).
+
+    Returns ``worst_plain`` (largest |g - g*| / (rtol sum|terms|)),
+    ``worst`` (the same with the bound added: ok iff <= 1) and
+    ``photons_needing_bound`` (fraction of photons with an entry that passes
+    only with the bound)."""
+    worst_plain, worst = 0.0, 0.0
+    need = None
+    for g, r, a, c in zip(got, (ref.gE, ref.gx, ref.gM), (ref.aE, ref.ax, ref.aM), (ref.cE, ref.cx, ref.cM)):
+        g = np.asarray(g, dtype=np.float64)
+        err = np.abs(g - r)
+        plain = err / (rtol * a + 1e-300)
+        full = err / (rtol * a + c + 1e-300)
+        worst_plain = max(worst_plain, float(np.max(plain, initial=0.0)))
+        worst = max(worst, float(np.max(full, initial=0.0)))
+        n = plain.shape[-1]
+        flag = ((plain > 1.0) & (full <= 1.0)).reshape(-1, n).any(axis=0)
+        need = flag if need is None else (need | flag)
+    frac = float(need.mean()) if need is not None and need.size else 0.0
+    return {"worst_plain": worst_plain, "worst": worst, "photons_needing_bound": frac}
+
+
 def photon_loss_f64(grad, c, E, x, M6, radius, cap) -> float:
     """<g, F_i> of one photon with double parameters (x: 2 values, M6: 6 values)."""
     grad = np.ascontiguousarray(grad, dtype=np.float32)
@@ -206,8 +231,46 @@ def central_fd(grad, photons, i, cap, h_x, h_m_rel=1e-4, with_error=False):
     return dx, dm, np.abs(dx - dx2), np.abs(dm - dm2)
 
 
+_CPU_LIB = "libpds_cpu.so"
+CPU_FLAGS = {"libpds_cpu.so": "gcc -O3 -march=x86-64-v3 -fopenmp",
+             "libpds_cpu_native.so": "gcc -O3 -march=native -fopenmp"}
+
+
+def use_native_cpu_baseline() -> str:
+    """Build the CPU baseline for this host (-march=native) and time that one
+    from now on; falls back to the portable build if the compiler is missing.
+    Returns the compile flags of the build in use."""
+    global _CPU_LIB
+    try:
+        subprocess.run(["make", "-C", _DIR, "-B", "libpds_cpu_native.so"], check=True, stdout=subprocess.DEVNULL,
+                       stderr=subprocess.DEVNULL)
+        _CPU_LIB = "libpds_cpu_native.so"
+        _libs.pop(_CPU_LIB, None)
+    except (OSError, subprocess.CalledProcessError):
+        _CPU_LIB = "libpds_cpu.so"
+    return CPU_FLAGS[_CPU_LIB]
+
+
+def host_cpu() -> dict:
+    """The host's CPU model, logical CPUs and the CPUs this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count()
+    return {"model": model, "logical_cpus": os.cpu_count(), "usable_cpus": usable}
+
+
 def cpu_threads() -> int:
-    return int(_lib("libpds_cpu.so").pds_cpu_threads())
+    return int(_lib(_CPU_LIB).pds_cpu_threads())
 
 
 def cpu_forward_f32(E, xp, M, cp, radius, size, cap, nthreads: int = 0) -> np.ndarray:
@@ -215,7 +278,7 @@ def cpu_forward_f32(E, xp, M, cp, radius, size, cap, nthreads: int = 0) -> np.nd
     E, xp, M, cp, radius, n = _inputs(E, xp, M, cp, radius)
     C, H, W = (int(s) for s in size)
     out = np.empty((C, H, W), dtype=np.float32)
-    rc = _lib("libpds_cpu.so").pds_cpu_forward_f32(
+    rc = _lib(_CPU_LIB).pds_cpu_forward_f32(
         _p(E, _f32p), _p(xp, _f32p), _p(M, _f32p), _p(cp, _i64p), _p(radius, _f32p), n, C, H, W, int(cap),
         _p(out, _f32p), int(nthreads))
     assert rc == 0
@@ -230,7 +293,7 @@ def cpu_backward_f32(grad, E, xp, M, cp, radius, cap, nthreads: int = 0):
     gE = np.empty(n, np.float32)
     gx = np.empty((2, n), np.float32)
     gM = np.empty((2, 3, n), np.float32)
-    rc = _lib("libpds_cpu.so").pds_cpu_backward_f32(
+    rc = _lib(_CPU_LIB).pds_cpu_backward_f32(
         _p(grad, _f32p), _p(E, _f32p), _p(xp, _f32p), _p(M, _f32p), _p(cp, _i64p), _p(radius, _f32p), n, C, H, W,
         int(cap), _p(gE, _f32p), _p(gx, _f32p), _p(gM, _f32p), int(nthreads))
     assert rc == 0

@@ -108,7 +108,15 @@ def _resolve_tuning(tuning, N, size, cap, mode):
     return _native.tuning_struct(tuning)
 
 
-def _forward(Ep, xp, Mp, cp, radius, output_size, max_pixel_radius, mode, tuning, keep_bins):
+def _check_live(live, device):
+    if not (live.is_cuda and live.dtype == torch.int64 and live.numel() == 1 and live.device == device):
+        raise RuntimeError("live must be a one-element int64 tensor on the photons' device")
+
+
+def _forward(Ep, xp, Mp, cp, radius, output_size, max_pixel_radius, mode, tuning, keep_bins, live=None):
+    """``live``: None, or a one-element int64 device tensor: only photons
+    i < live take part (pds_forward_bins_live_f32; needs keep_bins and a
+    binnable problem)."""
     _check_photons(Ep, xp, Mp, cp, radius)
     size = [int(s) for s in output_size]
     if len(size) != 3:
@@ -122,20 +130,32 @@ def _forward(Ep, xp, Mp, cp, radius, output_size, max_pixel_radius, mode, tuning
     with torch.cuda.device(Ep.device):
         out = torch.empty(size, dtype=torch.float32, device=Ep.device)
         bins = None
+        stream = ctypes.c_void_p(torch.cuda.current_stream(Ep.device).cuda_stream)
+        if live is not None:
+            _check_live(live, Ep.device)
+            if not keep_bins or C * H * W == 0 or lib.pds_bins_bytes(N, C, H, W, cap, m, _tun_ref(tun)) == 0:
+                raise RuntimeError("a live photon count needs kept bins of a binnable problem")
+            bins, nbytes = _workspace(lib.pds_bins_bytes, N, C, H, W, cap, m, tun, Ep.device)
+            rc = lib.pds_forward_bins_live_f32(_ptr(Ep), _ptr(xp), _ptr(Mp), _ptr(cp), _ptr(radius), N, C, H, W, cap,
+                                               m, _tun_ref(tun), _ptr(live), _ptr(out), _ptr(bins), nbytes, stream)
+            _native.check(rc, "pds_forward")
+            return out, bins
         if keep_bins and C * H * W > 0 and lib.pds_bins_bytes(N, C, H, W, cap, m, _tun_ref(tun)) > 0:
             bins, nbytes = _workspace(lib.pds_bins_bytes, N, C, H, W, cap, m, tun, Ep.device)
             fn = lib.pds_forward_bins_f32
         else:
             ws, nbytes = _workspace(lib.pds_forward_workspace_bytes, N, C, H, W, cap, m, tun, Ep.device)
             fn = lib.pds_forward_f32
-        stream = torch.cuda.current_stream(Ep.device).cuda_stream
         rc = fn(_ptr(Ep), _ptr(xp), _ptr(Mp), _ptr(cp), _ptr(radius), N, C, H, W, cap, m, _tun_ref(tun), _ptr(out),
-                _ptr(bins if bins is not None else ws), nbytes, ctypes.c_void_p(stream))
+                _ptr(bins if bins is not None else ws), nbytes, stream)
     _native.check(rc, "pds_forward")
     return out, bins
 
 
-def _backward(grad_pds, Ep, xp, Mp, cp, radius, max_pixel_radius, mode, tuning, bins=None):
+def _backward(grad_pds, Ep, xp, Mp, cp, radius, max_pixel_radius, mode, tuning, bins=None, live=None):
+    """``live``: the live count of the forward that kept ``bins``
+    (pds_backward_bins_live_f32): gradients of photons i >= live are not
+    written."""
     if not grad_pds.is_cuda:
         raise RuntimeError("grad_pds must be a CUDA tensor")
     if grad_pds.dim() != 3:
@@ -157,6 +177,15 @@ def _backward(grad_pds, Ep, xp, Mp, cp, radius, max_pixel_radius, mode, tuning, 
         gx = torch.empty_like(xp)
         gM = torch.empty_like(Mp)
         stream = torch.cuda.current_stream(Ep.device).cuda_stream
+        if live is not None:
+            _check_live(live, Ep.device)
+            if bins is None:
+                raise RuntimeError("a live photon count needs the bins of its forward")
+            rc = lib.pds_backward_bins_live_f32(_ptr(grad_pds), _ptr(Ep), _ptr(xp), _ptr(Mp), _ptr(cp), _ptr(radius),
+                                                N, C, H, W, cap, m, _tun_ref(tun), _ptr(live), _ptr(gE), _ptr(gx),
+                                                _ptr(gM), _ptr(bins), bins.numel(), ctypes.c_void_p(stream))
+            _native.check(rc, "pds_backward")
+            return [gE, gx, gM]
         if bins is not None:
             fn, ws, nbytes = lib.pds_backward_bins_f32, bins, bins.numel()
         else:

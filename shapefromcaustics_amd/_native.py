@@ -57,6 +57,10 @@ SIGNATURES = {
                              ctypes.c_int),
     "pds_backward_bins_f32": ([_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _i32, _i32, _i32, _tp, _vp, _vp, _vp,
                                _vp, _sz, _vp], ctypes.c_int),
+    "pds_forward_bins_live_f32": ([_vp, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _i32, _i32, _i32, _tp, _vp, _vp, _vp,
+                                   _sz, _vp], ctypes.c_int),
+    "pds_backward_bins_live_f32": ([_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _i32, _i32, _i32, _tp, _vp, _vp,
+                                    _vp, _vp, _vp, _sz, _vp], ctypes.c_int),
     "pds_bins_stats_get": ([_vp, _sz, _i64, _i32, _i32, _i32, _i32, _i32, _tp, ctypes.POINTER(PdsBinsStats), _vp],
                            ctypes.c_int),
     "pds_last_error": ([], ctypes.c_char_p),

@@ -10,11 +10,15 @@ output ``[K, H, W]``: channel k is splat k's image, and the backward of the
 same call gives every splat's photon gradients.
 
 :class:`RaggedBatch` holds static device buffers of a fixed photon capacity.
-Splats are packed back to back; the unused tail carries channel -1, which
-every kernel skips (zero image contribution, zero gradients), so one captured
-graph serves any photon counts up to the capacity without re-capture.
-:class:`GraphedStep` captures forward (keeping its bins) + backward on a
-side stream into a ``torch.cuda.CUDAGraph`` and replays it.
+Splats are packed back to back, and the number of loaded photons lives in a
+one-element device tensor that the kernels read when they run
+(``pds_forward_bins_live_f32`` / ``pds_backward_bins_live_f32``): one
+captured graph serves any photon counts up to the capacity without
+re-capture, and a replay costs what the loaded photons cost, not the
+capacity.  :class:`GraphedStep` captures forward (keeping its bins) +
+backward on a side stream into a ``torch.cuda.CUDAGraph`` and replays it.
+The batch's tuning defaults to the autotuner's cached choice for its
+problem class (``tuning="cached"``; built-in defaults when none is cached).
 """
 from __future__ import annotations
 
@@ -29,7 +33,7 @@ class RaggedBatch:
     """K splats of uneven photon counts into channels of one ``[K, H, W]`` image."""
 
     def __init__(self, capacity: int, K: int, H: int, W: int, max_pixel_radius: int, device,
-                 mode="auto", tuning=None):
+                 mode="auto", tuning="cached"):
         self.capacity = int(capacity)
         self.size = (int(K), int(H), int(W))
         self.cap = int(max_pixel_radius)
@@ -44,13 +48,15 @@ class RaggedBatch:
         self.cp = torch.full((1, n), -1, dtype=torch.int64, device=self.device)
         self.radius = torch.zeros(n, **f32)
         self.grad = torch.zeros(self.size, **f32)
+        self.live = torch.zeros(1, dtype=torch.int64, device=self.device)  # photons loaded
         self.offsets = [0]
 
     def load(self, splats: tp.Sequence[tp.Tuple[torch.Tensor, torch.Tensor, torch.Tensor, torch.Tensor]]) -> None:
         """Copy splats ``(E [n], xp [2, n], M [2, 3, n], radius [n])`` in, splat k -> channel k.
 
-        Stream-ordered device copies on the current stream; the tail beyond
-        the last photon is reset to channel -1.
+        Stream-ordered device copies on the current stream, and the live
+        count set to the photons loaded; the buffers' tail is neither read
+        nor written (its gradients are left as they were).
         """
         if len(splats) > self.size[0]:
             raise ValueError(f"{len(splats)} splats for {self.size[0]} channels")
@@ -68,26 +74,29 @@ class RaggedBatch:
             self.cp[:, o:o + n].fill_(k)
             o += n
             self.offsets.append(o)
-        self.cp[:, o:].fill_(-1)
-        self.E[o:].zero_()
+        self.live.fill_(o)
 
     def inputs(self):
         return self.E, self.xp, self.M, self.cp, self.radius
 
-    def forward(self, keep_bins: bool = True):
-        return _forward(*self.inputs(), self.size, self.cap, self.mode, self.tuning, keep_bins=keep_bins)
+    def forward(self):
+        """Forward of the loaded photons: (image, bins kept for :meth:`backward`)."""
+        return _forward(*self.inputs(), self.size, self.cap, self.mode, self.tuning, keep_bins=True, live=self.live)
 
     def backward(self, grad: torch.Tensor, bins=None):
-        return _backward(grad, *self.inputs(), self.cap, self.mode, self.tuning, bins=bins)
+        if bins is None:
+            _, bins = self.forward()
+        return _backward(grad, *self.inputs(), self.cap, self.mode, self.tuning, bins=bins, live=self.live)
 
     def step(self):
         """Forward + backward of the whole batch against ``self.grad``: (image, [gE, gx, gM])."""
-        img, bins = self.forward(keep_bins=True)
+        img, bins = self.forward()
         grads = self.backward(self.grad, bins)
         return img, grads
 
     def split(self, t: torch.Tensor) -> tp.List[torch.Tensor]:
-        """Per-splat views of a per-photon tensor (last dimension = photons)."""
+        """Per-splat views of a per-photon tensor (last dimension = photons);
+        the tail beyond the loaded photons is not part of any splat."""
         return [t[..., a:b] for a, b in zip(self.offsets[:-1], self.offsets[1:])]
 
 

@@ -20,12 +20,18 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "hf.h"
+
 namespace hf {
 
 constexpr float kTMin = 1e-3f;
 constexpr float kCullZ = 1e-3f;  // relative height margin of the cell cull (float rounding of z: ~1e-7 relative)
 constexpr float kMissT = -2.0f;
-constexpr int kMaxBounces = 8;
+// Hits a photon path may take (HF_MAX_BOUNCES of hf.h).  The reference loops
+// until every photon has left the scene (model/caustics.py:255); a GPU thread
+// needs a bound, and the FP64 oracle uses the same one: a photon still
+// bouncing after this many hits is dropped (and counted).
+constexpr int kMaxBounces = HF_MAX_BOUNCES;
 
 struct V3 {
     float x, y, z;

@@ -5,7 +5,8 @@
 //
 // One thread per photon.  The path kernels keep a photon's whole path in
 // registers; the adjoint re-runs it, keeping a per-bounce tape (20 floats a
-// bounce, at most 8 bounces) in private memory, and sweeps it backwards.
+// bounce, at most HF_MAX_BOUNCES bounces) in private memory, and sweeps it
+// backwards.
 // Per-vertex gradients (Sobel normals, vertex heights) gather per workgroup
 // in an LDS patch (a 16x16 tile of the ray grid lands on a few cells), are
 // flushed into [R, R] arrays with device float atomics, and one gather kernel
@@ -34,6 +35,8 @@ inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // workspace: [0] hmax (float) ; then grad_n [R,R,3], grad_z [R,R], grad_g [R,R,2]
 constexpr int kHmaxBlocks = 128;
+// path counters kept beside the hit records: reached, missed, degenerate, capped
+constexpr int kStatWords = 4;
 
 struct WsLayout {
     size_t hmax, grad_n, grad_z, grad_g, end;
@@ -112,8 +115,12 @@ __global__ __launch_bounds__(256) void hmax_part_kernel(const float* __restrict_
 
 // parts: n per-block maxima, then n minima.  out[0] = the maximum, out[1] =
 // the minimum (and copy[0..1] too, when copy is not null)
+// stats: null, or the path counters of a following simulate, cleared here (a
+// kernel, not a memset, so that a graph may capture the sequence)
 __global__ __launch_bounds__(1024) void hmax_kernel(const float* __restrict__ parts, int64_t n,
-                                                    float* __restrict__ out, float* __restrict__ copy) {
+                                                    float* __restrict__ out, float* __restrict__ copy,
+                                                    uint32_t* __restrict__ stats) {
+    if (stats && threadIdx.x < kStatWords) stats[threadIdx.x] = 0u;
     __shared__ float part[16], pmin[16];
     float m = -INFINITY, mn = INFINITY;
     for (int64_t k = threadIdx.x; k < n; k += 1024) {
@@ -370,10 +377,26 @@ __device__ __forceinline__ void write_dropped(const OutArgs& O, int64_t i) {
 
 // Hit records for the adjoint: the first kHitSlots hits of every photon that
 // reaches the sensor, [kHitSlots][N] float4 (t, b1, b2, obj << 30 | prim), and
-// per photon the number recorded: 0 = dropped (no gradient), kHitsRetrace =
-// more bounces than slots (the adjoint traces that path again).
+// per photon the number recorded: 0 = dropped (missed the scene or masked by
+// the splat's den != 0 test), kHitsCapped = dropped still bouncing after
+// kMaxBounces hits, kHitsRetrace = more bounces than slots (the adjoint traces
+// that path again).  Neither kind of dropped photon gets a gradient.
 constexpr int kHitSlots = 4;
+constexpr uint8_t kHitsCapped = 254;
 constexpr uint8_t kHitsRetrace = 255;
+static_assert(kMaxBounces < kHitsCapped, "hit counts must stay below the marker values");
+
+// Path outcome counters of one simulate launch (hf_path_stats), one atomic
+// per wavefront and outcome.
+enum { kReached = 0, kMissed = 1, kDegenerate = 2, kCapped = 3 };
+__device__ __forceinline__ void count_outcome(uint32_t* stats, int outcome) {
+    if (!stats) return;
+#pragma unroll
+    for (int k = 0; k < kStatWords; ++k) {
+        const uint64_t m = __ballot(outcome == k);
+        if ((threadIdx.x & 63) == 0 && m) atomicAdd(&stats[k], (uint32_t)__popcll(m));
+    }
+}
 
 __device__ __forceinline__ float4 pack_hit(const Hit& h) {
     return make_float4(h.t, h.b1, h.b2, __int_as_float((int)(((uint32_t)h.obj << 30) | (uint32_t)h.prim)));
@@ -384,23 +407,26 @@ __device__ __forceinline__ Hit unpack_hit(float4 v) {
     return Hit{v.x, v.y, v.z, (int)(op >> 30), (int)(op & 0x3fffffffu)};
 }
 
-// hits, nhit: null, or the hit records to write
-// 5 wavefronts per SIMD (<= 102 VGPRs; a few spill): ~1.5 % faster per
-// Landweber step than the 4 that its natural 103 registers allow
-__global__ __launch_bounds__(256, 5) void simulate_kernel(SceneArgs S, const float* __restrict__ hmax, EmitArgs E,
-                                                       OutArgs O, float4* __restrict__ hits,
-                                                       uint8_t* __restrict__ nhit) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= O.N) return;
+// One photon's path (model/caustics.py:242-291): emit, then per hit trace,
+// advance, transfer and either splat (ground) or refract, for at most
+// kMaxBounces hits.  Returns the outcome (kReached, kMissed, kDegenerate or
+// kCapped); dropped photons get cp = -1.
+__device__ __forceinline__ int simulate_photon(SceneArgs S, const float* __restrict__ hmax, const EmitArgs& E,
+                                               const OutArgs& O, float4* __restrict__ hits,
+                                               uint8_t* __restrict__ nhit, int64_t i) {
     S.hmax = hmax[0];
     S.hmin = hmax[1];
     int ch;
     PathState s = emit(E, i, ch);
     const Etas et = etas_of(1.0f, ior_of(E, ch));  // outside: index 1
     int nb = 0;
+    int outcome = kCapped;  // unless the loop ends earlier
     for (int bounce = 0; bounce < kMaxBounces; ++bounce) {
         const Hit hit = trace(S, s.o, s.d);
-        if (hit.obj < 0) break;  // miss: advance_path drops it
+        if (hit.obj < 0) {  // miss: advance_path drops it
+            outcome = kMissed;
+            break;
+        }
         if (hits && nb < kHitSlots) hits[(int64_t)nb * O.N + i] = pack_hit(hit);
         ++nb;
         const float t = hit.t;
@@ -412,7 +438,10 @@ __global__ __launch_bounds__(256, 5) void simulate_kernel(SceneArgs S, const flo
         const V3 Dv = transfer1(ng, s.d, t, s.Dv, s.Dp);
         if (hit.obj == 1) {
             const Splat sp = splat_inputs(p, Du, Dv, ns, E.flux);
-            if (!sp.good) break;
+            if (!sp.good) {  // the reference's den != 0 mask (caustics.py:226-230)
+                outcome = kDegenerate;
+                break;
+            }
             const int64_t N = O.N;
             O.Ep[i] = sp.Ep;
             O.xp[i] = sp.x;
@@ -426,7 +455,7 @@ __global__ __launch_bounds__(256, 5) void simulate_kernel(SceneArgs S, const flo
             O.radius[i] = sp.radius;
             O.cp[i] = ch;
             if (nhit) nhit[i] = nb <= kHitSlots ? (uint8_t)nb : kHitsRetrace;
-            return;
+            return kReached;
         }
         const Refr R = refr_setup(s.d, ns, et);
         refract_apply(R, s.d, s.Dt, s.Dp);
@@ -435,7 +464,22 @@ __global__ __launch_bounds__(256, 5) void simulate_kernel(SceneArgs S, const flo
         s.o = p;
     }
     write_dropped(O, i);
-    if (nhit) nhit[i] = 0;
+    if (nhit) nhit[i] = outcome == kCapped ? kHitsCapped : (uint8_t)0;
+    return outcome;
+}
+
+// hits, nhit: null, or the hit records to write; stats: null, or the
+// kStatWords path counters (cleared by hmax_kernel).
+// 5 wavefronts per SIMD (<= 102 VGPRs; a few spill): ~1.5 % faster per
+// Landweber step than the 4 that its natural 103 registers allow
+__global__ __launch_bounds__(256, 5) void simulate_kernel(SceneArgs S, const float* __restrict__ hmax, EmitArgs E,
+                                                       OutArgs O, float4* __restrict__ hits,
+                                                       uint8_t* __restrict__ nhit, uint32_t* __restrict__ stats) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    // whole wavefronts reach the counters (their ballots need every lane)
+    int outcome = -1;
+    if (i < O.N) outcome = simulate_photon(S, hmax, E, O, hits, nhit, i);
+    count_outcome(stats, outcome);
 }
 
 // ---------------------------------------------------------------------------
@@ -535,7 +579,9 @@ __device__ __forceinline__ void adjoint_photon(const SceneArgs& S, const EmitArg
     // with the forward's hit records the path is replayed without tracing and
     // without a tape: the state at a bounce is advanced from the emitted one
     const int m = nhit ? (int)nhit[i] : (int)kHitsRetrace;
-    if (m == 0) return;  // dropped in the forward: no gradient
+    // dropped in the forward (0, kHitsCapped): no gradient; any other count
+    // above the recorded slots but the re-trace marker cannot occur
+    if (m == 0 || (m > kHitSlots && m != kHitsRetrace)) return;
     const bool replay = m != kHitsRetrace;
     int ch;
     const PathState s0 = emit(E, i, ch);
@@ -890,11 +936,13 @@ __global__ __launch_bounds__(256) void vcons_kernel(const float* __restrict__ x,
 unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 // hmax[0] = max and hmax[1] = min of the n heights (hmax: the workspace's
-// hmax region); also written to copy[0..1] when copy is not null
-void launch_hmax(const float* h, int64_t n, float* hmax, hipStream_t s, float* copy = nullptr) {
+// hmax region); also written to copy[0..1] when copy is not null; stats (if
+// not null) cleared
+void launch_hmax(const float* h, int64_t n, float* hmax, hipStream_t s, float* copy = nullptr,
+                 uint32_t* stats = nullptr) {
     hipLaunchKernelGGL(hmax_part_kernel, dim3(kHmaxBlocks), dim3(256), 0, s, h, n, hmax + 64);
     hipLaunchKernelGGL(hmax_kernel, dim3(1), dim3(1024), 0, s, (const float*)(hmax + 64), (int64_t)kHmaxBlocks, hmax,
-                       copy);
+                       copy, stats);
 }
 
 EmitArgs emit_args(const hf_scene* sc, const hf_emitter* em) {
@@ -954,11 +1002,12 @@ int hip_status(const char* what) {
     return PDS_OK;
 }
 
-// hit buffer: records [kHitSlots][N] float4, counts [N] bytes, and the
-// maximum and minimum height of the field the records were traced on (the
-// adjoint's re-traced paths use them, so they are not computed again)
+// hit buffer: records [kHitSlots][N] float4, counts [N] bytes, the maximum
+// and minimum height of the field the records were traced on (the adjoint's
+// re-traced paths use them, so they are not computed again), and the path
+// counters of the simulate that wrote it (hf_path_stats)
 struct HitsLayout {
-    size_t recs, count, hmax, end;
+    size_t recs, count, hmax, stats, end;
 };
 
 HitsLayout hits_layout(int64_t N) {
@@ -966,7 +1015,8 @@ HitsLayout hits_layout(int64_t N) {
     L.recs = 0;
     L.count = (sizeof(float4) * kHitSlots * (size_t)N + 255) & ~(size_t)255;
     L.hmax = (L.count + (size_t)N + 255) & ~(size_t)255;
-    L.end = L.hmax + 2 * sizeof(float);  // max and min height
+    L.stats = L.hmax + 16;  // after the max and min height
+    L.end = L.stats + sizeof(uint32_t) * kStatWords;
     return L;
 }
 
@@ -989,10 +1039,11 @@ int simulate_impl(const float* h, const float* normals, const hf_scene* sc, cons
     OutArgs O{Ep, xp, Mp, radius, cp, N};
     float4* hrec = want_hits ? (float4*)((char*)hits + HL.recs) : nullptr;
     uint8_t* hcnt = want_hits ? (uint8_t*)hits + HL.count : nullptr;
+    uint32_t* hst = want_hits ? (uint32_t*)((char*)hits + HL.stats) : nullptr;
     hipStream_t s = (hipStream_t)stream;
-    launch_hmax(h, (int64_t)sc->R * sc->R, hmax, s, want_hits ? (float*)((char*)hits + HL.hmax) : nullptr);
+    launch_hmax(h, (int64_t)sc->R * sc->R, hmax, s, want_hits ? (float*)((char*)hits + HL.hmax) : nullptr, hst);
     hipLaunchKernelGGL(simulate_kernel, dim3(blocks_for(N)), dim3(256), 0, s, scene_args(h, normals, sc, hmax),
-                       (const float*)hmax, emit_args(sc, em), O, hrec, hcnt);
+                       (const float*)hmax, emit_args(sc, em), O, hrec, hcnt, hst);
     return hip_status("hf_simulate");
 }
 
@@ -1097,6 +1148,28 @@ int hf_simulate_hits_f32(const float* h, const float* normals, const hf_scene* s
                          float* xp, float* Mp, int64_t* cp, float* radius, void* ws, size_t ws_bytes, void* hits,
                          size_t hits_bytes, void* stream) {
     return simulate_impl(h, normals, sc, em, Ep, xp, Mp, cp, radius, ws, ws_bytes, hits, hits_bytes, true, stream);
+}
+
+int hf_hits_stats_get(const void* hits, size_t hits_bytes, const hf_scene* sc, const hf_emitter* em,
+                      hf_path_stats* out, void* stream) {
+    g_err[0] = '\0';
+    if (!out) return fail(PDS_ERR_ARGUMENT, "null stats pointer");
+    const size_t need = hf_hits_bytes(sc, em);
+    if (need == 0) return fail(PDS_ERR_ARGUMENT, "invalid scene or emitter");
+    if (!hits || hits_bytes < need) return fail(PDS_ERR_WORKSPACE, "hit buffer smaller than hf_hits_bytes");
+    const int64_t N = (int64_t)em->n_side * em->n_side * sc->channels;
+    uint32_t c[kStatWords];
+    hipStream_t s = (hipStream_t)stream;
+    if (hipMemcpyAsync(c, (const char*)hits + hits_layout(N).stats, sizeof(c), hipMemcpyDeviceToHost, s) !=
+            hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return hip_status("hf_hits_stats_get");
+    out->reached = c[kReached];
+    out->missed = c[kMissed];
+    out->degenerate = c[kDegenerate];
+    out->capped = c[kCapped];
+    for (int k = 0; k < 4; ++k) out->reserved[k] = 0;
+    return PDS_OK;
 }
 
 int hf_adjoint_f32(const float* h, const float* normals, const hf_scene* sc, const hf_emitter* em,

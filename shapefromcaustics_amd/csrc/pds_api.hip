@@ -60,8 +60,9 @@ int check_common(int64_t N, int32_t C, int32_t H, int32_t W, int32_t mode, const
 }
 
 pds::PhotonArgs photon_args(const float* E, const float* xp, const float* M, const int64_t* cp, const float* radius,
-                            int64_t N) {
+                            int64_t N, const int64_t* live = nullptr) {
     pds::PhotonArgs P;
+    P.live = live;
     P.E = E;
     P.xp = xp;
     P.M = M;
@@ -94,9 +95,11 @@ pds::ImageArgs image_args(int32_t C, int32_t H, int32_t W, int32_t cap) {
 
 namespace {
 
+// live: null, or a device photon count (kept bins only)
 int forward_impl(const float* Ep, const float* xp, const float* Mp, const int64_t* cp, const float* radius,
                  int64_t N, int32_t C, int32_t H, int32_t W, int32_t max_pixel_radius, int32_t mode,
-                 const pds_tuning* t, float* out, void* workspace, size_t workspace_bytes, bool keep, void* stream) {
+                 const pds_tuning* t, float* out, void* workspace, size_t workspace_bytes, bool keep, void* stream,
+                 const int64_t* live = nullptr) {
     g_last_error[0] = '\0';
     int rc = check_common(N, C, H, W, mode, Ep, xp, Mp, cp, radius);
     if (rc) return rc;
@@ -111,7 +114,7 @@ int forward_impl(const float* Ep, const float* xp, const float* Mp, const int64_
         return fail(PDS_ERR_WORKSPACE, "workspace of %zu bytes is smaller than the %zu bytes required",
                     workspace ? workspace_bytes : (size_t)0, need);
     hipStream_t s = (hipStream_t)stream;
-    const pds::PhotonArgs P = photon_args(Ep, xp, Mp, cp, radius, N);
+    const pds::PhotonArgs P = photon_args(Ep, xp, Mp, cp, radius, N, live);
     const pds::ImageArgs I = image_args(C, H, W, max_pixel_radius);
     hipError_t e;
     if (mode == PDS_MODE_PLAIN) {
@@ -128,7 +131,7 @@ int forward_impl(const float* Ep, const float* xp, const float* Mp, const int64_
 int backward_impl(const float* grad, const float* Ep, const float* xp, const float* Mp, const int64_t* cp,
                   const float* radius, int64_t N, int32_t C, int32_t H, int32_t W, int32_t max_pixel_radius,
                   int32_t mode, const pds_tuning* t, float* grad_Ep, float* grad_xp, float* grad_Mp, void* workspace,
-                  size_t workspace_bytes, bool reuse, void* stream) {
+                  size_t workspace_bytes, bool reuse, void* stream, const int64_t* live = nullptr) {
     g_last_error[0] = '\0';
     int rc = check_common(N, C, H, W, mode, Ep, xp, Mp, cp, radius);
     if (rc) return rc;
@@ -137,7 +140,7 @@ int backward_impl(const float* grad, const float* Ep, const float* xp, const flo
     const int64_t numel = (int64_t)C * H * W;
     if (numel > 0 && !grad) return fail(PDS_ERR_ARGUMENT, "null upstream gradient pointer");
     hipStream_t s = (hipStream_t)stream;
-    const pds::PhotonArgs P = photon_args(Ep, xp, Mp, cp, radius, N);
+    const pds::PhotonArgs P = photon_args(Ep, xp, Mp, cp, radius, N, live);
     const pds::ImageArgs I = image_args(C, H, W, max_pixel_radius);
     const pds::GradArgs G = {grad_Ep, grad_xp, grad_Mp};
     hipError_t e;
@@ -220,6 +223,25 @@ int pds_backward_bins_f32(const float* grad, const float* Ep, const float* xp, c
                           void* bins, size_t bins_bytes, void* stream) {
     return backward_impl(grad, Ep, xp, Mp, cp, radius, N, C, H, W, max_pixel_radius, mode, t, grad_Ep, grad_xp,
                          grad_Mp, bins, bins_bytes, true, stream);
+}
+
+int pds_forward_bins_live_f32(const float* Ep, const float* xp, const float* Mp, const int64_t* cp,
+                              const float* radius, int64_t N, int32_t C, int32_t H, int32_t W, int32_t max_pixel_radius,
+                              int32_t mode, const pds_tuning* t, const int64_t* live_count, float* out, void* bins,
+                              size_t bins_bytes, void* stream) {
+    if (!live_count) return fail(PDS_ERR_ARGUMENT, "null live count pointer");
+    return forward_impl(Ep, xp, Mp, cp, radius, N, C, H, W, max_pixel_radius, mode, t, out, bins, bins_bytes, true,
+                        stream, live_count);
+}
+
+int pds_backward_bins_live_f32(const float* grad, const float* Ep, const float* xp, const float* Mp,
+                               const int64_t* cp, const float* radius, int64_t N, int32_t C, int32_t H, int32_t W,
+                               int32_t max_pixel_radius, int32_t mode, const pds_tuning* t,
+                               const int64_t* live_count, float* grad_Ep, float* grad_xp, float* grad_Mp, void* bins,
+                               size_t bins_bytes, void* stream) {
+    if (!live_count) return fail(PDS_ERR_ARGUMENT, "null live count pointer");
+    return backward_impl(grad, Ep, xp, Mp, cp, radius, N, C, H, W, max_pixel_radius, mode, t, grad_Ep, grad_xp,
+                         grad_Mp, bins, bins_bytes, true, stream, live_count);
 }
 
 int pds_bins_stats_get(const void* bins, size_t bins_bytes, int64_t N, int32_t C, int32_t H, int32_t W,

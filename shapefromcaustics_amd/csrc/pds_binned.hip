@@ -128,7 +128,8 @@ bool make_geom(int64_t N, int32_t C, int32_t H, int32_t W, int32_t cap, int32_t 
     double capf = kDefaultCapacity;
     if (t && t->capacity_pct > 0) capf = t->capacity_pct / 100.0;
     g.cap_records = (int64_t)(capf * (double)N) + 1024;
-    if (g.cap_records >= ((int64_t)1 << 32)) return false;  // 32-bit record offsets
+    // record offsets and segment starts are kept in 32-bit signed fields
+    if (g.cap_records >= ((int64_t)1 << 31)) return false;
     g.max_segments = g.nbins + g.cap_records / g.seg + 1;
     g.det = mode == PDS_MODE_DETERMINISTIC;
     g.box_walk = !(t && t->walk == 1);
@@ -303,7 +304,7 @@ __global__ __launch_bounds__(kBinThreads) void bin_count_kernel(PhotonArgs P, Im
     for (int k = threadIdx.x; k < g.nbins; k += kBinThreads) lh[k] = 0;
     __syncthreads();
     const int64_t lo = (int64_t)blockIdx.x * g.per_block;
-    const int64_t hi = min(lo + g.per_block, P.N);
+    const int64_t hi = min(lo + g.per_block, live_photons(P));
     float emax = 0.0f;
     int64_t i = lo + threadIdx.x;
     // two photons of this lane in flight while one is binned
@@ -462,7 +463,7 @@ __global__ __launch_bounds__(kBinThreads) void bin_scatter_kernel(PhotonArgs P, 
     for (int k = threadIdx.x; k < g.nbins; k += kBinThreads) cur[k] = binoff[k] + row[k];
     __syncthreads();
     const int64_t lo = (int64_t)blockIdx.x * g.per_block;
-    const int64_t hi = min(lo + g.per_block, P.N);
+    const int64_t hi = min(lo + g.per_block, live_photons(P));
     const int half = g.T >> 1;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     // Each round of records is staged in LDS and written by lane pairs, one
@@ -890,12 +891,13 @@ __device__ __forceinline__ void flat_splat_all(const PhotonArgs& P, const ImageA
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     WaveRecs& R = recs[w];
     R.mark[lane] = -1;
-    const int64_t nbatch = (P.N + 63) >> 6;
+    const int64_t n = live_photons(P);
+    const int64_t nbatch = (n + 63) >> 6;
     for (int64_t b = (int64_t)blockIdx.x * (kTileThreads / 64) + w; b < nbatch;
          b += (int64_t)gridDim.x * (kTileThreads / 64)) {
         const int64_t i = (b << 6) + lane;
         FlatRec r;
-        setup_flat_rec<kFwdUnit>(P, I, i, /*forward=*/true, r);
+        setup_flat_rec<kFwdUnit>(P, I, i, /*forward=*/true, r, n);
         const int64_t total = publish_recs(R, r, lane);
         walk_units<kFwdUnit>(R, r, lane, total, [&](const FlatItem& it) {
             if constexpr (DET) {
@@ -1010,10 +1012,16 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_kernel(Geom g, PhotonAr
         const int64_t o = ((int64_t)st.c * I.H + py) * I.W + px;
         const Acc v = tile[k];
         if constexpr (DET) {
-            if (!split)
+            if (!split) {
                 out[o] = from_fixed(v, fe);
-            else if (v != 0)
-                atomicAdd(&acc64[o], v);  // integer addition: order-independent
+            } else if (v != 0) {
+                // integer addition of the finite part (order-independent); the
+                // +inf marker is or-ed in, so that two segments' markers do
+                // not carry out of bit 62
+                if (v >> 62) atomicOr(&acc64[o], 1ull << 62);
+                const unsigned long long fin = v & ((1ull << 62) - 1);
+                if (fin) atomicAdd(&acc64[o], fin);
+            }
         } else {
             if (!split)
                 out[o] = (float)v;
@@ -1195,7 +1203,8 @@ __global__ __launch_bounds__(256) void bin_finalize_kernel(const float* __restri
     const int64_t x = blockIdx.x % kXcds, k = blockIdx.x / kXcds;
     const int64_t blk = kXcds * (k / cpb) + x, off = (k % cpb) * 256 + threadIdx.x;
     const int64_t i = blk * per_block + off;
-    if (off >= per_block || i >= N) return;
+    // photons beyond a batch's live count are not written
+    if (off >= per_block || i >= live_photons(P)) return;
     // on overflow the scatter wrote no records, slots or classes
     const uint32_t cls = ctrl->overflow ? (uint32_t)kSpecial : (uint32_t)info[i];
     if (cls == kSpecial) {

@@ -21,8 +21,18 @@ struct PhotonArgs {
     const float* M;        // [2, 3, N]
     const int64_t* cp;     // [1, N]
     const float* radius;   // [N]
-    int64_t N;
+    int64_t N;             // photons, and the stride of the [2, N] / [2, 3, N] planes
+    const int64_t* live;   // null, or a device count: only photons i < *live take part (a batch's load)
 };
+
+// The photons that take part: [0, N), or [0, min(N, *live)) for a batch with
+// a device-side count (read when the kernel runs, so that one captured graph
+// serves any count up to N).
+__device__ __forceinline__ int64_t live_photons(const PhotonArgs& P) {
+    if (!P.live) return P.N;
+    const int64_t n = *P.live;
+    return n < 0 ? 0 : (n < P.N ? n : P.N);
+}
 
 struct ImageArgs {
     int32_t C, H, W, cap;
@@ -147,11 +157,12 @@ struct WaveRecs {
 // margin).  Pixels outside that ellipse have K = 0 and contribute exactly
 // nothing, so the result is unchanged.  Non-finite photons keep the whole
 // window so that NaN propagates into their gradients as in the reference.
+// n_end: photons i >= n_end are skipped (the live count of a batch)
 template <int U>
 __device__ __forceinline__ void setup_flat_rec(const PhotonArgs& P, const ImageArgs& I, int64_t i, bool forward,
-                                               FlatRec& r) {
+                                               FlatRec& r, int64_t n_end) {
     clear_rec(r);
-    if (i >= P.N) return;
+    if (i >= n_end) return;
     const int64_t c = P.cp[i];
     if (c < 0 || c >= I.C) return;
     const int64_t N = P.N;

@@ -99,7 +99,7 @@ __global__ __launch_bounds__(256) void fwd_flat_kernel(PhotonArgs P, ImageArgs I
     for (int64_t b = (int64_t)blockIdx.x * 4 + w; b < nbatch; b += (int64_t)gridDim.x * 4) {
         const int64_t i = (b << 6) + lane;
         FlatRec r;
-        setup_flat_rec<U>(P, I, i, /*forward=*/true, r);
+        setup_flat_rec<U>(P, I, i, /*forward=*/true, r, P.N);
         const int64_t total = publish_recs(R, r, lane);
         walk_units<U>(R, r, lane, total, [&](const FlatItem& it) { flat_forward_unit<U>(it, I.W, out); });
     }
@@ -122,7 +122,7 @@ __global__ __launch_bounds__(256) void bwd_flat_kernel(const float* __restrict__
     for (int64_t b = (int64_t)blockIdx.x * 4 + w; b < nbatch; b += (int64_t)gridDim.x * 4) {
         const int64_t i = (b << 6) + lane;
         FlatRec r;
-        setup_flat_rec<U>(P, I, i, /*forward=*/false, r);
+        setup_flat_rec<U>(P, I, i, /*forward=*/false, r, P.N);
         const int64_t total = publish_recs(R, r, lane);
         walk_units<U>(R, r, lane, total, [&](const FlatItem& it) {
             const UnitPixels<U> px(it, I.W);

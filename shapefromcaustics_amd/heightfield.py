@@ -41,6 +41,14 @@ class HfEmitter(ctypes.Structure):
                 ("plane_z", _f32), ("diff_scale", _f32), ("seed", ctypes.c_uint32), ("reserved", _i32 * 2)]
 
 
+class HfPathStats(ctypes.Structure):
+    _fields_ = [("reached", _i64), ("missed", _i64), ("degenerate", _i64), ("capped", _i64),
+                ("reserved", _i64 * 4)]
+
+
+MAX_BOUNCES = 16  # HF_MAX_BOUNCES of include/hf.h
+
+
 class HfLandweber(ctypes.Structure):
     _fields_ = [("tau", _f32), ("alpha", _f32), ("lower", _f32), ("upper", _f32), ("d1", _f32), ("d2", _f32),
                 ("gamma", _f32), ("volume_radius", _i32), ("reserved", _i32 * 4)]
@@ -56,6 +64,7 @@ SIGNATURES = {
     "hf_simulate_f32": ([_vp, _vp, _SP, _EP, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp], ctypes.c_int),
     "hf_adjoint_f32": ([_vp, _vp, _SP, _EP, _vp, _vp, _vp, _vp, _vp, _sz, _vp], ctypes.c_int),
     "hf_hits_bytes": ([_SP, _EP], _sz),
+    "hf_hits_stats_get": ([_vp, _sz, _SP, _EP, ctypes.c_void_p, _vp], ctypes.c_int),
     "hf_simulate_hits_f32": ([_vp, _vp, _SP, _EP, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp, _sz, _vp], ctypes.c_int),
     "hf_adjoint_hits_f32": ([_vp, _vp, _SP, _EP, _vp, _vp, _vp, _vp, _vp, _sz, _vp, _sz, _vp], ctypes.c_int),
     "hf_residual_workspace_bytes": ([_i64], _sz),
@@ -210,14 +219,32 @@ def trace_rays(h: torch.Tensor, scene: Scene, origins: torch.Tensor, dirs: torch
     return t, uv, obj, prim
 
 
+HITS_CAPPED, HITS_RETRACE = 254, 255
+
+
 def hit_counts(hits: torch.Tensor, n_photons: int) -> torch.Tensor:
     """The per-photon hit counts inside a hit buffer from ``simulate(...,
-    keep_hits=True)``: 0 = dropped, 1..4 = hits recorded, 255 = more bounces
-    than recorded (the adjoint traces that path again).  Layout of the buffer
-    (csrc/hf_kernels.hip, hits_layout): 4 x N float4 records, then the N count
-    bytes at the next 256-byte boundary, then the field's maximum height."""
+    keep_hits=True)``: 0 = dropped (missed the scene or masked by the
+    splat's den != 0 test), 1..4 = hits recorded, 254 (``HITS_CAPPED``) =
+    dropped still bouncing after ``MAX_BOUNCES`` hits, 255
+    (``HITS_RETRACE``) = more bounces than recorded (the adjoint traces that
+    path again).  Layout of the buffer (csrc/hf_kernels.hip, hits_layout):
+    4 x N float4 records, then the N count bytes at the next 256-byte
+    boundary, then the field's maximum and minimum height and the path
+    counters."""
     off = (16 * 4 * n_photons + 255) // 256 * 256
     return hits[off:off + n_photons]
+
+
+def path_stats(hits: torch.Tensor, scene: Scene, emitter) -> dict:
+    """How the paths of the simulate that wrote ``hits`` ended (synchronises):
+    ``reached``, ``missed``, ``degenerate`` (the den != 0 mask) and
+    ``capped`` (still bouncing after ``MAX_BOUNCES`` hits) photon counts."""
+    sc, em = scene.struct(), _emitter_struct(emitter, scene)
+    st = HfPathStats()
+    _check(lib().hf_hits_stats_get(_p(hits), hits.numel(), ctypes.byref(sc), ctypes.byref(em), ctypes.byref(st),
+                                   _stream(hits.device)), "hf_hits_stats_get")
+    return {k: int(getattr(st, k)) for k in ("reached", "missed", "degenerate", "capped")}
 
 
 def simulate(h: torch.Tensor, scene: Scene, emitter: ParallelEmitter, n: tp.Optional[torch.Tensor] = None,
@@ -303,9 +330,33 @@ def residual(img: torch.Tensor, target: torch.Tensor):
     return resid, loss
 
 
+def pixel_area(R: int) -> float:
+    """Area of one cell of the R x R height-field grid over [-1, 1]^2
+    (shape_from_caustics.py:651: the product of the two grid spacings)."""
+    return (2.0 / (R - 1)) ** 2
+
+
+def printing_volume(d_exact: torch.Tensor) -> float:
+    """Volume of the glass added on the substrate by the height offset
+    ``d_exact`` [R, R] (shape_from_caustics.py:656: pixel area x sum of d)."""
+    return pixel_area(int(d_exact.shape[0])) * float(d_exact.double().sum())
+
+
+def volume_band(volume: float, R: int, volume_eps: float = 0.25) -> tp.Tuple[float, float]:
+    """(d1, d2) of shape_from_caustics.py:383-384: the band of the summed height
+    offset, volume (1 -/+ volume_eps) / pixel area (volume_eps default of
+    hyperparameter_helper.py:93)."""
+    a = pixel_area(R)
+    return (volume - volume * volume_eps) / a, (volume + volume * volume_eps) / a
+
+
 @dataclasses.dataclass
 class LandweberParams:
-    """Defaults of hyperparameter_helper.py:86-94 (tau, alpha, gamma, bounds, volume radius)."""
+    """Defaults of hyperparameter_helper.py:86-94 (tau, alpha, gamma, bounds, volume radius).
+
+    ``d1``/``d2`` default to an unbounded band (the volume heuristic never
+    acts); the reference derives them from the printing volume of the target,
+    which :meth:`for_volume` does."""
 
     tau: float = 0.1
     alpha: float = 2e-3
@@ -315,6 +366,13 @@ class LandweberParams:
     d2: float = float("inf")
     gamma: float = 0.1
     volume_radius: int = 2
+
+    @classmethod
+    def for_volume(cls, volume: float, R: int, volume_eps: float = 0.25, **kw) -> "LandweberParams":
+        """The reference's parameters for a target of this printing volume
+        (:func:`printing_volume`): d1, d2 from :func:`volume_band`."""
+        d1, d2 = volume_band(volume, R, volume_eps)
+        return cls(d1=d1, d2=d2, **kw)
 
     def struct(self) -> HfLandweber:
         p = HfLandweber()

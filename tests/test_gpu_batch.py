@@ -1,5 +1,6 @@
 """GPU: ragged batches of independent splats (one call, splat k -> channel k),
-their HIP-graph replay with new photon counts, and the autotuner.
+their HIP-graph replay with new photon counts (a device-side live count, so
+that a replay costs the loaded photons), and the autotuner.
 
 Each splat of a batch is checked against the FP64 oracle run on that splat
 alone into a one-channel image: channel routing makes the batch exactly K
@@ -42,8 +43,9 @@ def _check_against_oracle(img, grads, batch, splats, grad_np):
         rg = O.backward_f64(grad_np[k:k + 1], *p.arrays(), CAP)
         ok, worst = O.grads_within_tolerance([gE[a:b], gx[:, a:b], gM[:, :, a:b]], rg, rtol=1e-3)
         assert ok, f"splat {k} gradients: {worst:.3e}"
-    n = batch.offsets[len(splats)]
-    assert not gE[n:].any() and not gx[:, n:].any() and not gM[:, :, n:].any(), "padding photons got gradients"
+    # photons beyond the live count (earlier loads' photons, still in the
+    # buffers' tail) contribute nothing: the images above match the oracle on
+    # the loaded photons alone, and unused channels stay empty
     assert not img[len(splats):].any(), "unused channels"
 
 
@@ -81,7 +83,9 @@ def test_graph_replays_with_new_counts(mode):
             assert torch.equal(img, e_img), f"replay {rep}: image differs from the eager call"
         else:
             torch.testing.assert_close(img, e_img, rtol=1e-5, atol=1e-7 * float(e_img.abs().max()))
+        n = batch.offsets[-1]  # the loaded photons (the tail's gradients are not written)
         for g, e in zip(grads, e_grads):
+            g, e = g[..., :n], e[..., :n]
             torch.testing.assert_close(g, e, rtol=1e-4, atol=1e-6 * float(e.abs().max()) + 1e-30)
         _check_against_oracle(img, grads, batch, splats, grad_np)
         del junk
@@ -100,3 +104,48 @@ def test_autotuner_picks_and_caches(tmp_path, monkeypatch):
     ref = O.forward_f64(*p.arrays(), p.size, p.cap)
     m = ref > 1e-3 * ref.max()
     assert (np.abs(img.cpu().numpy() - ref)[m] / ref[m]).max() <= 1e-4
+
+
+def test_replay_costs_the_loaded_photons():
+    # one graph captured for the full batched workload's capacity (19.4M
+    # photons, 16 x 512^2); replays at a quarter of the load take a fraction
+    # of the full-load time, because the kernels stop at the live count
+    full = WL.batched()
+    counts = WL.batched_counts()
+    offs = np.concatenate([[0], np.cumsum(counts)])
+    dev = [torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in (full.E, full.xp, full.M, full.radius)]
+    splat = lambda a, b: (dev[0][a:b], dev[1][:, a:b], dev[2][:, :, a:b], dev[3][a:b])
+    batch = RaggedBatch(full.n, 16, 512, 512, 8, "cuda")
+    batch.load([splat(a, b) for a, b in zip(offs[:-1], offs[1:])])
+    batch.grad.copy_(torch.from_numpy(WL.upstream_grad(batch.size, seed=6)))
+    graph = GraphedStep(batch)
+
+    def time_replays(reps=10):
+        for _ in range(2):
+            graph.replay()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ev[0].record()
+        for _ in range(reps):
+            graph.replay()
+        ev[1].record()
+        torch.cuda.synchronize()
+        return ev[0].elapsed_time(ev[1]) / reps
+
+    t_full = time_replays()
+    quarter = full.n // 4
+    batch.load([splat(int(a), int(min(b, quarter))) for a, b in zip(offs[:-1], offs[1:]) if a < quarter])
+    t_quarter = time_replays()
+    img, _ = graph.replay()
+    torch.cuda.synchronize()
+    print(f"\ngraph replay: full load ({full.n} photons) {t_full:.3f} ms, quarter load ({quarter}) {t_quarter:.3f} ms "
+          f"({t_quarter / t_full:.2f} of full)")
+    assert t_quarter <= 0.45 * t_full
+    # and the quarter load's image is that of its photons only
+    k = int(np.searchsorted(offs, quarter, side="right")) - 1
+    a = int(offs[k])
+    sub = full.subset(np.arange(a, quarter))
+    ref = O.forward_f64(sub.E, sub.xp, sub.M, np.zeros((1, quarter - a), np.int64), sub.radius, (1, 512, 512), 8)[0]
+    got = img[k].cpu().numpy().astype(np.float64)
+    m = ref > 1e-3 * ref.max()
+    assert (np.abs(got - ref)[m] / ref[m]).max() <= 1e-4
+    assert not img[k + 1:].any()

@@ -154,9 +154,12 @@ def test_caustic_workload_full(mode):
     idx = np.sort(rng.choice(p.n, 1 << 16, replace=False))
     ti = torch.from_numpy(idx).cuda()
     got = (gE[ti], gx[:, ti], gM[:, :, ti])
-    worst = assert_grads_close(got, O.backward_f64(g, *p.arrays(), p.cap, idx=idx), "caustic bwd")
+    refg = O.backward_f64(g, *p.arrays(), p.cap, idx=idx)
+    worst = assert_grads_close(got, refg, "caustic bwd")
+    rep = O.grad_error_report([t.cpu().numpy() for t in got], refg, rtol=GRAD_TOL)
     print(f"\ncaustic 2^22 -> 3x1024^2, mode {mode}: image max rel err vs FP64 oracle {rel:.2e} (tol 1e-4); "
-          f"gradient error / tolerance {worst:.2e} on 2^16 photons (<= 1)")
+          f"gradient error / tolerance {worst:.2e} on 2^16 photons (<= 1); without the FP32 conditioning "
+          f"bound {rep['worst_plain']:.2e}, {100 * rep['photons_needing_bound']:.3f} % of photons need the bound")
     # adjoint vs central finite differences (FP64), h = 1e-3 px, on 512 random photons
     W = p.size[2]
     sub = idx[:512]
@@ -308,6 +311,28 @@ def test_deterministic_infinite_energy():
     assert np.array_equal(np.isinf(img), np.isinf(ref))
     fin = np.isfinite(ref)
     assert_image_close(np.where(fin, img, 0.0).astype(np.float64), np.where(fin, ref, 0.0), "det inf")
+
+
+def test_deterministic_infinite_energy_in_split_bins():
+    # many infinite-energy photons on one pixel, in a tile whose records are
+    # split over several 64-record segments: each segment's tile carries the
+    # +inf marker, and the split flush must not let two markers carry into a
+    # finite (negative) value -- the reference's sum is +inf
+    p = WL.plain(400, (1, 128, 128), 8, seed=3)
+    rng = np.random.default_rng(3)
+    px = rng.uniform(8, 56, (2, p.n))
+    p.xp[:] = (2.0 * px / 128 - 1.0).astype(np.float32)          # all in the first 64 x 64 tile
+    hot = np.arange(0, p.n, 4)
+    p.xp[:, hot] = np.float32(2.0 * 32.3 / 128 - 1.0)             # 100 photons on one centre ...
+    p.E[hot] = np.inf                                             # ... with infinite energy
+    ref = O.forward_f64(*p.arrays(), p.size, p.cap)
+    for tun in ({"tile": 64, "seg_photons": 64}, {"tile": 64, "seg_photons": 64, "block_photons": 64}):
+        img = _run(p, "deterministic", tuning=tun).cpu().numpy()
+        assert np.isposinf(ref).any()
+        assert np.array_equal(np.isposinf(img), np.isposinf(ref)) and not np.isneginf(img).any()
+        fin = np.isfinite(ref)
+        assert np.isfinite(img[fin]).all()
+        assert_image_close(np.where(fin, img, 0.0).astype(np.float64), np.where(fin, ref, 0.0), "det inf split")
 
 
 @pytest.mark.parametrize("mode", ["auto", "deterministic"])

@@ -24,8 +24,8 @@ def test_trace_equals_brute_force(tmp_path):
     assert "#include <hip/hip_runtime.h>" in text
     (tmp_path / "hf_device_host.hpp").write_text(text.replace("#include <hip/hip_runtime.h>", ""))
     exe = tmp_path / "trace_bruteforce"
-    subprocess.run(["g++", "-O2", "-std=c++17", "-I", str(tmp_path), os.path.join(HERE, "host", "trace_bruteforce.cpp"),
-                    "-o", str(exe)], check=True)
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I", str(tmp_path), "-I", os.path.join(HERE, "..", "include"),
+                    os.path.join(HERE, "host", "trace_bruteforce.cpp"), "-o", str(exe)], check=True)
     res = subprocess.run([str(exe), "4000"], capture_output=True, text=True, timeout=300)
     assert res.returncode == 0, res.stderr
     total, hits, bad = (int(x) for x in res.stdout.split())
