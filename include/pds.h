@@ -100,6 +100,25 @@ int pds_backward_f32(const float* grad, const float* Ep, const float* xp, const 
                      float* grad_Ep, float* grad_xp, float* grad_Mp, void* workspace,
                      size_t workspace_bytes, void* stream);
 
+/* ---- forward without a workspace ----------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): pds_forward_f32 with the same arguments and results,
+ * whose scratch is instead a fixed device array of PDS_STATIC_SCRATCH_BYTES
+ * inside the library.  Each GPU the library runs on holds its own array
+ * until the process ends.  A problem whose pds_forward_workspace_bytes
+ * exceeds that size returns PDS_ERR_CAPACITY (2^22 photons into 3 x 1024^2
+ * fit with capacity_pct <= 190).  The call allocates nothing.  Every call on
+ * one GPU shares that GPU's array, which keeps the last call's contents
+ * (each call initialises what it reads): calls that overlap on one GPU --
+ * on other streams or from other host threads -- race, and the caller must
+ * serialise them. */
+#define PDS_STATIC_SCRATCH_BYTES ((size_t)256 << 20)
+int pds_forward_noalloc_f32(const float* Ep, const float* xp, const float* Mp, const int64_t* cp,
+                            const float* radius, int64_t N, int32_t C, int32_t H, int32_t W,
+                            int32_t max_pixel_radius, int32_t mode, const pds_tuning* t, float* out,
+                            void* stream);
+
 /* ---- forward/backward pair sharing one binning -------------------------
  * The autograd path runs the forward and then the backward of the same
  * photons.  pds_forward_bins_f32 leaves the photons' tile binning in the

@@ -21,6 +21,7 @@ PDS_ERR_HIP = -3
 PDS_ERR_CAPACITY = -4
 
 MODES = {"auto": 0, "plain": 1, "binned": 2, "deterministic": 3}
+STATIC_SCRATCH_BYTES = 256 << 20  # PDS_STATIC_SCRATCH_BYTES of include/pds.h
 
 
 class PdsTuning(ctypes.Structure):
@@ -52,6 +53,8 @@ SIGNATURES = {
     "pds_forward_f32": ([_vp, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _i32, _i32, _i32, _tp, _vp, _vp, _sz, _vp], ctypes.c_int),
     "pds_backward_f32": ([_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _i32, _i32, _i32, _tp, _vp, _vp, _vp, _vp,
                           _sz, _vp], ctypes.c_int),
+    "pds_forward_noalloc_f32": ([_vp, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _i32, _i32, _i32, _tp, _vp, _vp],
+                                ctypes.c_int),
     "pds_bins_bytes": ([_i64, _i32, _i32, _i32, _i32, _i32, _tp], _sz),
     "pds_forward_bins_f32": ([_vp, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _i32, _i32, _i32, _tp, _vp, _vp, _sz, _vp],
                              ctypes.c_int),

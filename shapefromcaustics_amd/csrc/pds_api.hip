@@ -13,6 +13,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 
 #include "pds.h"
 #include "pds_binned.hpp"
@@ -170,6 +171,45 @@ int backward_impl(const float* grad, const float* Ep, const float* xp, const flo
     return PDS_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Scratch of pds_forward_noalloc_f32: one fixed array per GPU, part of this
+// library's device code object, so that the entry point needs no allocation.
+// In host code `scratch` names a host-side stand-in, not device memory: its
+// device address comes from hipGetSymbolAddress, once per device (each GPU
+// holds its own copy at its own address), and only that address is used.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[PDS_STATIC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's scratch address (resolved on its first use there), or
+// null with the error recorded.
+void* device_scratch() {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        hip_fail(e, "hipGetDevice");
+        return nullptr;
+    }
+    if (dev < 0 || dev >= kMaxDevices) {
+        fail(PDS_ERR_ARGUMENT, "device %d is beyond the %d devices the static scratch supports", dev, kMaxDevices);
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr) {
+            hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
+}
+
 }  // namespace
 
 extern "C" {
@@ -200,6 +240,24 @@ int pds_forward_f32(const float* Ep, const float* xp, const float* Mp, const int
                     const pds_tuning* t, float* out, void* workspace, size_t workspace_bytes, void* stream) {
     return forward_impl(Ep, xp, Mp, cp, radius, N, C, H, W, max_pixel_radius, mode, t, out, workspace,
                         workspace_bytes, false, stream);
+}
+
+int pds_forward_noalloc_f32(const float* Ep, const float* xp, const float* Mp, const int64_t* cp,
+                            const float* radius, int64_t N, int32_t C, int32_t H, int32_t W, int32_t max_pixel_radius,
+                            int32_t mode, const pds_tuning* t, float* out, void* stream) {
+    g_last_error[0] = '\0';
+    int rc = check_common(N, C, H, W, mode, Ep, xp, Mp, cp, radius);
+    if (rc) return rc;
+    const size_t need = pds_forward_workspace_bytes(N, C, H, W, max_pixel_radius, mode, t);
+    if (need > PDS_STATIC_SCRATCH_BYTES)
+        return fail(PDS_ERR_CAPACITY,
+                    "pds_forward_noalloc_f32: this problem needs %zu bytes of scratch, more than the %zu bytes of the "
+                    "static scratch (use pds_forward_f32 with a workspace, or a lower capacity_pct)",
+                    need, (size_t)PDS_STATIC_SCRATCH_BYTES);
+    void* ws = nullptr;
+    if (need > 0 && (ws = device_scratch()) == nullptr) return PDS_ERR_HIP;
+    return forward_impl(Ep, xp, Mp, cp, radius, N, C, H, W, max_pixel_radius, mode, t, out, ws,
+                        need > 0 ? (size_t)PDS_STATIC_SCRATCH_BYTES : 0, false, stream);
 }
 
 int pds_forward_bins_f32(const float* Ep, const float* xp, const float* Mp, const int64_t* cp, const float* radius,
