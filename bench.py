@@ -10,6 +10,13 @@ before the timed region.  One "step" = pds_forward + pds_backward of the whole
 batch.  Prints ONE JSON line on stdout; progress goes to stderr.
 
     python bench.py [--steps K] [--warmup W] [--mode auto|plain|binned|deterministic]
+
+The same line carries, under "configs", the batched inner simulations (16
+ragged splats under one HIP graph vs 16 sequential plain-atomic launches),
+one Landweber step of height-field reconstruction and the 4 x 2^24-photon
+scale-out (atomic and deterministic, photons regenerated), each with its own
+comparator and roofline; --workload batched|landweber|scaleout prints one of
+them alone.
 """
 from __future__ import annotations
 
@@ -43,17 +50,18 @@ def make_workload(name):
     elif name == "plain":
         p = WL.plain()
         desc = "plain: 2^18 photons -> 1x256x256 fp32, cap 8, uniform"
-    elif name == "batched":
-        p = WL.batched()
-        desc = "batched: 16 ragged splats (logU(2^18,2^22) photons, 19365832 total) -> 16x512x512 fp32, cap 8"
     else:
         raise SystemExit(f"unknown workload {name}")
     return p, desc
 
 
 def cpu_baseline(p, grad, budget_s=12.0):
-    """OpenMP FP32 host implementation of the same formulae on the same inputs."""
+    """OpenMP FP32 host implementation of the same formulae on the same inputs,
+    built for this host (-march=native) and run on every CPU this process may
+    use (the box's share for one GPU)."""
     from oracle import pds_oracle as O
+    flags = O.use_native_cpu_baseline()
+    host = O.host_cpu()
     threads = O.cpu_threads()
     times = []
     t_start = time.perf_counter()
@@ -66,7 +74,9 @@ def cpu_baseline(p, grad, budget_s=12.0):
     t = times[len(times) // 2]
     return {"value": p.n / t, "unit": "photons/s", "cores": threads, "kind": "port",
             "sample": f"the full benchmark batch ({p.n} photons, fwd+bwd), median of {len(times)} runs "
-                      f"on {threads} OpenMP threads of the host"}
+                      f"on {threads} OpenMP threads of the host",
+            "host_cpu": host["model"], "host_logical_cpus": host["logical_cpus"],
+            "usable_cpus": host["usable_cpus"], "build": flags}
 
 
 def load_traffic(kernel, workload):
@@ -80,20 +90,11 @@ def load_traffic(kernel, workload):
         return None
 
 
-def bench_landweber(args):
-    """One pixel-Landweber iteration of height-field reconstruction, end to end
-    on the GPU: Sobel normals, 2048^2 parallel rays traced and refracted
-    through a 256^2 height field (32 Gaussian bumps, 1 mm base, n = 1.5) to a
-    sensor 50 mm below, splatted into 1024^2 (cap 12); loss 1/2 |I - T|^2
-    against the image of a differently seeded field; PDS adjoint, path
-    adjoint (replaying the forward's ray hits) and the masked shrinkage +
-    volume update."""
+def landweber_fields():
+    """The config-4 height fields: 256^2 heights, 32 Gaussian bumps (amplitude
+    <= 0.5 mm, sigma 0.3-2 mm over 10 x 10 mm) on a 1 mm base."""
     import numpy as np
-    import torch
-
-    from shapefromcaustics_amd import heightfield as HF
-
-    R, n_side, S, cap, sz, base = 256, 2048, 1024, 12, -10.0, 0.2
+    R, base = 256, 0.2
 
     def field(seed):
         rng = np.random.Generator(np.random.PCG64(seed))
@@ -104,40 +105,56 @@ def bench_landweber(args):
         h = np.full((R, R), base)
         for k in range(32):
             h += a[k] * np.exp(-((X - mu[k, 0]) ** 2 + (Y - mu[k, 1]) ** 2) / (2 * sg[k] ** 2))
-        return torch.tensor(h, dtype=torch.float32, device="cuda")
+        return h
+    return R, base, field
 
+
+def bench_landweber(steps, warmup, with_plain=True):
+    """One pixel-Landweber iteration of height-field reconstruction, end to end
+    on the GPU: Sobel normals, 2048^2 parallel rays traced and refracted
+    through a 256^2 height field (32 Gaussian bumps, 1 mm base, n = 1.5) to a
+    sensor 50 mm below, splatted into 1024^2 (cap 12); loss 1/2 |I - T|^2
+    against the image of a differently seeded field; PDS adjoint, path
+    adjoint (replaying the forward's ray hits) and the masked shrinkage +
+    volume update, with the volume band the reference derives from the
+    target's printing volume (shape_from_caustics.py:383-384).  Returns a dict
+    (ms/step, rays/s, stage split, the same step with the plain-atomic splat)."""
+    import torch
+
+    from shapefromcaustics_amd import heightfield as HF
+    from shapefromcaustics_amd.PhotonDifferentialSplatting import _backward, _forward
+
+    R, base, field = landweber_fields()
+    n_side, S, cap, sz = 2048, 1024, 12, -10.0
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     sc, em = HF.Scene(R, sz, (1.5,)), HF.ParallelEmitter(n_side, 2.0, 1.0)
     size = (1, S, S)
-    target = HF.CausticImage.apply(field(5), sc, em, size, cap).detach()
-    d = (field(4) - base).contiguous()
+    h_target = torch.tensor(field(5), dtype=torch.float32, device=dev)
+    target = HF.CausticImage.apply(h_target, sc, em, size, cap).detach()
+    d0 = torch.tensor(field(4) - base, dtype=torch.float32, device=dev).contiguous()
+    d = d0.clone()
     mask = torch.ones_like(d)
-    params = HF.LandweberParams()
+    params = HF.LandweberParams.for_volume(HF.printing_volume(h_target - base), R)
     N = n_side * n_side
 
-    d0 = d.clone()
+    def timed_steps(mode):
+        def step():
+            # every timed iteration starts from the same field (a 256^2 copy),
+            # so each measures the same physical problem
+            d.copy_(d0)
+            return HF.landweber_step(d, base, target, sc, em, size, cap, mask, params, mode=mode)
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
 
-    def step():
-        # every timed iteration starts from the same field (a 256^2 copy), so
-        # each measures the same physical problem
-        d.copy_(d0)
-        return HF.landweber_step(d, base, target, sc, em, size, cap, mask, params)
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-    t0 = time.perf_counter()
-    ev[0].record()
-    for _ in range(args.steps):
-        loss, img, gh = step()
-    ev[1].record()
-    torch.cuda.synchronize()
-    wall = (time.perf_counter() - t0) / args.steps
-    gpu = ev[0].elapsed_time(ev[1]) * 1e-3 / args.steps
+    wall = timed_steps("auto")
     # stage split on one extra step
-    from shapefromcaustics_amd.PhotonDifferentialSplatting import _backward, _forward
     e = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
     d.copy_(d0)
     h = (d + base).contiguous()
@@ -155,27 +172,38 @@ def bench_landweber(args):
     HF.landweber_update(d, gh, mask, params)
     e[5].record()
     torch.cuda.synchronize()
+    stats = HF.path_stats(hits, sc, em)
     names = ["normals+simulate", "splat+residual", "splat adjoint", "path adjoint", "update"]
     stages = {k: round(e[i].elapsed_time(e[i + 1]) * 1e3, 1) for i, k in enumerate(names)}
-    log(f"[bench] landweber step {wall * 1e3:.3f} ms (GPU {gpu * 1e3:.3f} ms); stages (us) {stages}; "
-        f"loss {float(lossv):.4e}; photons reaching the sensor {int((cp >= 0).sum())} of {N}")
-    line = {
-        "metric": "Landweber steps/s (height-field reconstruction iteration, rays/s in config)",
-        "value": 1.0 / wall, "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": wall * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic (seeded Gaussian-bump height fields)",
-        "config": {"workload": "landweber: 256^2 height field, 2048^2 parallel rays -> 1x1024x1024, cap 12, "
-                               "sensor 50 mm, n 1.5", "rays": N, "rays_per_s": N / wall, "stages_us": stages},
+    n_sensor = int((cp >= 0).sum())
+    splat_bytes = 132 * N + 8 * S * S  # the PDS pair over the step's photon buffers (reference layout)
+    splat_us = stages["splat+residual"] + stages["splat adjoint"]
+    plain = timed_steps("plain") if with_plain else None
+    log(f"[bench] landweber step {wall * 1e3:.3f} ms; stages (us) {stages}; loss {float(lossv):.4e}; "
+        f"photons reaching the sensor {n_sensor} of {N}; paths {stats}; volume band "
+        f"[{params.d1:.1f}, {params.d2:.1f}]" + (f"; plain-atomic splat {plain * 1e3:.3f} ms" if plain else ""))
+    return {
+        "workload": "landweber: 256^2 height field, 2048^2 parallel rays -> 1x1024x1024, cap 12, sensor 50 mm, "
+                    "n 1.5, volume band of the target (reference default step)",
+        "ms_per_step": round(wall * 1e3, 4), "steps_per_s": 1.0 / wall, "rays": N, "rays_per_s": N / wall,
+        "photons_on_sensor": n_sensor, "path_outcomes": stats, "stages_us": stages,
+        "volume_band": [params.d1, params.d2],
+        "splat_roofline": {"bound": "hbm", "achieved": round(splat_bytes / (splat_us * 1e-6) / 1e9, 1),
+                           "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(splat_bytes / (splat_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+                           "algorithmic_bytes": splat_bytes, "what": "PDS forward + adjoint stages of the step"},
+        "plain_atomic_ms_per_step": None if plain is None else round(plain * 1e3, 4),
+        "speedup_vs_plain": None if plain is None else round(plain / wall, 3),
     }
-    print(json.dumps(line), flush=True)
 
 
-def bench_scaleout(args):
+def bench_scaleout(steps, warmup, with_plain=True):
     """Memory-bound scale-out on one GPU: 4 views x 2^24 caustic-line photons
     into 12 x 2048 x 2048 (cap 16).  A step generates the photons (counter-
     based source), splats them keeping the bins, drops them, regenerates them
     for the backward (recomputation instead of storage) and runs the adjoint.
-    Timed in the atomic and the deterministic mode."""
+    Timed in the atomic and the deterministic mode; the plain-atomic
+    comparator keeps its photons (no generation).  Returns a dict."""
     import torch
 
     from shapefromcaustics_amd.generate import ScaleOut
@@ -195,40 +223,135 @@ def bench_scaleout(args):
 
     res = {}
     for mode in ("auto", "deterministic"):
-        for _ in range(args.warmup):
+        for _ in range(warmup):
             step(mode)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for _ in range(args.steps):
+        for _ in range(steps):
             step(mode)
         torch.cuda.synchronize()
-        res[mode] = (time.perf_counter() - t0) / args.steps
+        res[mode] = (time.perf_counter() - t0) / steps
         log(f"[bench] scaleout {mode}: {res[mode] * 1e3:.2f} ms/step ({so.n / res[mode]:.3e} photons/s)")
     plain = None
-    if not args.no_plain:
+    if with_plain:
         ph = so.photons()
-        for _ in range(1):
-            img = _forward(*ph, so.size, so.cap, "plain", None, keep_bins=False)[0]
-            _backward(grad, *ph, so.cap, "plain", None)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        img = _forward(*ph, so.size, so.cap, "plain", None, keep_bins=False)[0]
+        _forward(*ph, so.size, so.cap, "plain", None, keep_bins=False)
         _backward(grad, *ph, so.cap, "plain", None)
         torch.cuda.synchronize()
-        plain = time.perf_counter() - t0
+        reps = 2
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            _forward(*ph, so.size, so.cap, "plain", None, keep_bins=False)
+            _backward(grad, *ph, so.cap, "plain", None)
+        torch.cuda.synchronize()
+        plain = (time.perf_counter() - t0) / reps
+        del ph
         log(f"[bench] scaleout plain-atomic (photons kept, no generation): {plain * 1e3:.2f} ms/step")
     wall = res["auto"]
-    line = {
-        "metric": "photons/sec fwd+bwd, scale-out (4 views x 2^24 -> 12x2048x2048, cap 16, photons regenerated)",
-        "value": so.n / wall, "unit": "photons/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": wall * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic (counter-based caustic-line photons, N(0,1) upstream gradient)",
-        "config": {"workload": "scaleout: 4 views x 2^24 photons -> 12x2048x2048 fp32, cap 16, caustic lines; "
-                               "photons generated on the GPU and regenerated for the backward",
-                   "photons": so.n, "deterministic_ms_per_step": res["deterministic"] * 1e3,
-                   "plain_atomic_ms_per_step": None if plain is None else plain * 1e3},
+    nbytes = 132 * so.n + 8 * so.size[0] * so.size[1] * so.size[2]
+    return {
+        "workload": "scaleout: 4 views x 2^24 photons -> 12x2048x2048 fp32, cap 16, caustic lines; photons "
+                    "generated on the GPU and regenerated for the backward",
+        "photons": so.n, "ms_per_step": round(wall * 1e3, 4), "photons_per_s": so.n / wall,
+        "deterministic_ms_per_step": round(res["deterministic"] * 1e3, 4),
+        "deterministic_photons_per_s": so.n / res["deterministic"],
+        "roofline": {"bound": "hbm", "achieved": round(nbytes / wall / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(nbytes / wall / 1e9 / HBM_PEAK_GBS, 4),
+                     "algorithmic_bytes": nbytes, "what": "whole step, stored-photon byte convention"},
+        "plain_atomic_ms_per_step": None if plain is None else round(plain * 1e3, 4),
+        "speedup_vs_plain": None if plain is None else round(plain / wall, 3),
     }
-    print(json.dumps(line), flush=True)
+
+
+def bench_batched(steps, warmup, with_plain=True, tune=True):
+    """Batched inner simulations: 16 splats of logU(2^18, 2^22) photons
+    (19,365,832, seed 3) into 16 x 512^2, cap 8, as one forward + backward
+    replayed from a HIP graph (RaggedBatch + GraphedStep) with the
+    autotuner's tile, segment and block choice for this batch, against the
+    reference's way: 16 sequential plain-atomic forward + backward launches,
+    one per image.  Also a replay at a quarter of the load (same graph).
+    Returns a dict."""
+    import numpy as np
+    import torch
+
+    from shapefromcaustics_amd import autotune
+    from shapefromcaustics_amd import workloads as WL
+    from shapefromcaustics_amd.batch import GraphedStep, RaggedBatch
+    from shapefromcaustics_amd.PhotonDifferentialSplatting import _backward, _forward
+
+    torch.cuda.set_device(0)
+    p = WL.batched()
+    K, side, cap = p.size[0], p.size[1], p.cap
+    offs = np.concatenate([[0], np.cumsum(WL.batched_counts(K))])
+    dev = [torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in (p.E, p.xp, p.M, p.radius)]
+    splat = lambda a, b: (dev[0][a:b], dev[1][:, a:b], dev[2][:, :, a:b], dev[3][a:b])
+    grad = torch.from_numpy(WL.upstream_grad(p.size, seed=4)).cuda()
+    tuning = None
+    if tune:
+        E, xp, M, radius = dev
+        cp = torch.from_numpy(np.ascontiguousarray(p.cp)).cuda()
+        best = autotune.tune(E, xp, M, cp, radius, p.size, cap, grad=grad, reps=3, warmup=1, save=False)
+        tuning = {k: best[k] for k in ("tile", "seg_photons", "block_photons", "capacity_pct")}
+        del cp
+    batch = RaggedBatch(p.n, K, side, side, cap, "cuda", tuning=tuning)
+    batch.load([splat(int(a), int(b)) for a, b in zip(offs[:-1], offs[1:])])
+    batch.grad.copy_(grad)
+    graph = GraphedStep(batch)
+
+    def time_replays(reps):
+        for _ in range(warmup):
+            graph.replay()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ev[0].record()
+        for _ in range(reps):
+            graph.replay()
+        ev[1].record()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps, ev[0].elapsed_time(ev[1]) * 1e-3 / reps
+
+    wall, gpu_t = time_replays(steps)
+    quarter = p.n // 4
+    batch.load([splat(int(a), int(min(b, quarter))) for a, b in zip(offs[:-1], offs[1:]) if a < quarter])
+    wall_q, _ = time_replays(steps)
+    plain = None
+    if with_plain:
+        parts = []
+        for k in range(K):
+            a, b = int(offs[k]), int(offs[k + 1])
+            parts.append((*[t.contiguous() for t in splat(a, b)],
+                          torch.zeros((1, b - a), dtype=torch.int64, device="cuda"), grad[k:k + 1].contiguous()))
+
+        def run_plain():
+            for e, x, m, r, c, g in parts:
+                _forward(e, x, m, c, r, (1, side, side), cap, "plain", None, keep_bins=False)
+                _backward(g, e, x, m, c, r, cap, "plain", None)
+        run_plain()
+        torch.cuda.synchronize()
+        reps = max(2, steps // 4)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            run_plain()
+        torch.cuda.synchronize()
+        plain = (time.perf_counter() - t0) / reps
+    nbytes = 132 * p.n + 8 * K * side * side
+    log(f"[bench] batched graph replay {wall * 1e3:.3f} ms ({p.n / wall:.3e} photons/s), quarter load "
+        f"{wall_q * 1e3:.3f} ms; tuning {tuning}" +
+        (f"; {K} sequential plain-atomic launches {plain * 1e3:.3f} ms" if plain else ""))
+    return {
+        "workload": "batched: 16 ragged splats (logU(2^18,2^22) photons, 19365832 total) -> 16x512x512 fp32, cap 8, "
+                    "one forward + backward replayed from a HIP graph, autotuned",
+        "photons": p.n, "ms_per_step": round(wall * 1e3, 4), "photons_per_s": p.n / wall,
+        "graph_gpu_ms": round(gpu_t * 1e3, 4), "tuning": tuning,
+        "quarter_load_ms_per_step": round(wall_q * 1e3, 4), "quarter_load_ratio": round(wall_q / wall, 3),
+        "roofline": {"bound": "hbm", "achieved": round(nbytes / wall / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(nbytes / wall / 1e9 / HBM_PEAK_GBS, 4),
+                     "algorithmic_bytes": nbytes, "what": "whole replay (forward + backward of the batch)"},
+        "plain_atomic_ms_per_step": None if plain is None else round(plain * 1e3, 4),
+        "plain_what": f"{K} sequential plain-atomic forward + backward launches, one per image",
+        "speedup_vs_plain": None if plain is None else round(plain / wall, 3),
+    }
 
 
 def summarize_profile(path, factor_fetch=1.0, factor_write=1.0):
@@ -290,8 +413,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain-atomic comparator")
     ap.add_argument("--stateless", action="store_true",
                     help="time pds_forward + pds_backward (the backward bins again) instead of the PDS pairing")
-    ap.add_argument("--graph", action="store_true",
-                    help="capture the step in a HIP graph and time its replays (default for --workload batched)")
+    ap.add_argument("--graph", action="store_true", help="capture the step in a HIP graph and time its replays")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the batched, Landweber and scale-out sub-measurements of the default run")
     ap.add_argument("--eager", action="store_true", help="never use a HIP graph")
     ap.add_argument("--tuned", action="store_true", help="use the autotuner's cached tuning for this problem")
     ap.add_argument("--summarize", metavar="DB", help="print a Markdown summary of a rocprofv3 result and exit")
@@ -300,8 +424,6 @@ def main():
     args = ap.parse_args()
     if args.summarize:
         return summarize_profile(args.summarize, args.fetch_factor, args.write_factor)
-    if args.workload == "batched" and not args.eager:
-        args.graph = True
     if args.eager:
         args.graph = False
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -309,10 +431,17 @@ def main():
               "all-reduce is the design, not built); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
-    if args.workload == "landweber":
-        return bench_landweber(args)
-    if args.workload == "scaleout":
-        return bench_scaleout(args)
+    if args.workload in ("landweber", "scaleout", "batched"):
+        # one workload's own line (the default caustic run carries all three as sub-objects)
+        fn = {"landweber": bench_landweber, "scaleout": bench_scaleout, "batched": bench_batched}[args.workload]
+        d = fn(args.steps, args.warmup, with_plain=not args.no_plain)
+        unit_v = ("steps/s", d["steps_per_s"]) if args.workload == "landweber" else ("photons/s", d["photons_per_s"])
+        line = {"metric": f"{args.workload}: {unit_v[0]}", "value": unit_v[1], "unit": unit_v[0], "n_gpus": 1,
+                "steps": args.steps, "warmup": args.warmup, "ms_per_step": d["ms_per_step"],
+                "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+                "data": "synthetic", "config": {"workload": d["workload"]}, "detail": d}
+        print(json.dumps(line), flush=True)
+        return
     if args.workload == "copy":
         # calibration for HBM byte counters: 5 device-to-device copies of 1 GiB
         # (1048576 KiB read and written per copy)
@@ -336,7 +465,6 @@ def main():
     torch.cuda.set_device(dev)
     log(f"[bench] generating workload {args.workload}")
     p, desc = make_workload(args.workload)
-    offsets = np.concatenate([[0], np.cumsum(np.bincount(p.cp[0], minlength=p.size[0]))])
     grad_np = WL.upstream_grad(p.size, seed=2)
     E, xp, M, cp, radius = p.to_torch(dev)
     grad = torch.from_numpy(grad_np).to(dev)
@@ -422,32 +550,8 @@ def main():
 
     plain = None
     if not args.no_plain and args.mode != "plain":
-        if args.workload == "batched":
-            # the reference's way: one splat after another, each into its own image
-            parts = []
-            for k in range(p.size[0]):
-                a, b = int(offsets[k]), int(offsets[k + 1])
-                parts.append((E[a:b].contiguous(), xp[:, a:b].contiguous(), M[:, :, a:b].contiguous(),
-                              torch.zeros((1, b - a), dtype=torch.int64, device=dev), radius[a:b].contiguous(),
-                              grad[k:k + 1].contiguous()))
-
-            def run_plain(ev):
-                if ev is not None:
-                    ev[0].record()
-                outs = [_forward(e, x, m, c, r, (1, H, W), p.cap, "plain", None, keep_bins=False)[0]
-                        for e, x, m, c, r, _ in parts]
-                if ev is not None:
-                    ev[1].record()
-                for e, x, m, c, r, g in parts:
-                    _backward(g, e, x, m, c, r, p.cap, "plain", None)
-                if ev is not None:
-                    ev[2].record()
-                return outs
-            pw, pf, pb = timed(run_plain, max(3, args.steps // 4), 1)
-            label = f"{p.size[0]} sequential plain-atomic launches"
-        else:
-            pw, pf, pb = timed(lambda ev: step("plain", ev), max(3, args.steps // 4), 2)
-            label = "plain-atomic comparator"
+        pw, pf, pb = timed(lambda ev: step("plain", ev), max(3, args.steps // 4), 2)
+        label = "plain-atomic comparator"
         plain = {"value": N / pw, "ms_per_step": pw * 1e3, "fwd_us": pf * 1e6, "bwd_us": pb * 1e6, "what": label}
         log(f"[bench] {label}: {pw * 1e3:.3f} ms/step")
 
@@ -476,6 +580,18 @@ def main():
         cpu = cpu_baseline(p, grad_np)
         log(f"[bench] CPU: {cpu['value']:.3e} photons/s on {cpu['cores']} threads")
 
+    configs = None
+    if args.workload == "caustic" and not args.no_configs:
+        # the other configurations' own lines, each with its comparator and roofline
+        del E, xp, M, cp, radius, grad
+        torch.cuda.empty_cache()
+        configs = {}
+        for name, fn, k, w in (("batched", bench_batched, 10, 2), ("landweber", bench_landweber, 10, 2),
+                               ("scaleout", bench_scaleout, 5, 1)):
+            log(f"[bench] config {name}")
+            configs[name] = fn(min(args.steps, k), min(args.warmup, w), with_plain=not args.no_plain)
+            torch.cuda.empty_cache()
+
     line = {
         "metric": "photons/sec fwd+bwd @2^22 photons->1024^2 (% HBM roofline), batched vs plain",
         "value": N / wall,
@@ -499,6 +615,7 @@ def main():
         "plain_atomic": plain,
         "speedup_vs_plain": None if plain is None else round(plain["ms_per_step"] / (wall * 1e3), 3),
         "speedup_vs_cpu": None if cpu is None else round((N / wall) / cpu["value"], 1),
+        "configs": configs,
     }
     print(json.dumps(line), flush=True)
 

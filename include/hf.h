@@ -24,7 +24,7 @@
  * hits (the reference loops until every photon has left the scene,
  * model/caustics.py:255; a GPU thread needs a bound): a photon still
  * bouncing after HF_MAX_BOUNCES hits is dropped, and the hit-recording
- * simulate counts such photons (hf_hits_stats_get).
+ * simulate marks such photons (counted by hf_hits_stats_get).
  *
  * Conventions as in pds.h: device pointers, caller-owned buffers and
  * workspace, a hipStream_t passed as void*, 0 or a negative pds_status.
@@ -129,9 +129,10 @@ int hf_adjoint_f32(const float* h, const float* normals, const hf_scene* scene, 
  * order of the float atomic sums. */
 size_t hf_hits_bytes(const hf_scene* scene, const hf_emitter* emitter);
 
-/* How the paths of the hf_simulate_hits_f32 that wrote `hits` ended: read
- * from the buffer (synchronises `stream`; a diagnostic, not for the hot
- * loop).  reached + missed + degenerate + capped = N.  No reference
+/* How the paths of the hf_simulate_hits_f32 that wrote `hits` ended,
+ * counted from the per-photon records on the device (writes only the
+ * buffer's counter words; synchronises `stream`: a diagnostic, not for the
+ * hot loop).  reached + missed + degenerate + capped = N.  No reference
  * counterpart (the reference drops such photons silently). */
 typedef struct hf_path_stats {
     int64_t reached;     /* reached the sensor with a valid splat (cp >= 0) */
@@ -140,7 +141,7 @@ typedef struct hf_path_stats {
     int64_t capped;      /* still bouncing after HF_MAX_BOUNCES hits: dropped */
     int64_t reserved[4];
 } hf_path_stats;
-int hf_hits_stats_get(const void* hits, size_t hits_bytes, const hf_scene* scene, const hf_emitter* emitter,
+int hf_hits_stats_get(void* hits, size_t hits_bytes, const hf_scene* scene, const hf_emitter* emitter,
                       hf_path_stats* out, void* stream);
 int hf_simulate_hits_f32(const float* h, const float* normals, const hf_scene* scene, const hf_emitter* emitter,
                          float* Ep, float* xp, float* Mp, int64_t* cp, float* radius, void* workspace,

@@ -35,7 +35,7 @@ inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // workspace: [0] hmax (float) ; then grad_n [R,R,3], grad_z [R,R], grad_g [R,R,2]
 constexpr int kHmaxBlocks = 128;
-// path counters kept beside the hit records: reached, missed, degenerate, capped
+// path counters (hf_hits_stats_get): reached, missed, degenerate, capped
 constexpr int kStatWords = 4;
 
 struct WsLayout {
@@ -115,12 +115,8 @@ __global__ __launch_bounds__(256) void hmax_part_kernel(const float* __restrict_
 
 // parts: n per-block maxima, then n minima.  out[0] = the maximum, out[1] =
 // the minimum (and copy[0..1] too, when copy is not null)
-// stats: null, or the path counters of a following simulate, cleared here (a
-// kernel, not a memset, so that a graph may capture the sequence)
 __global__ __launch_bounds__(1024) void hmax_kernel(const float* __restrict__ parts, int64_t n,
-                                                    float* __restrict__ out, float* __restrict__ copy,
-                                                    uint32_t* __restrict__ stats) {
-    if (stats && threadIdx.x < kStatWords) stats[threadIdx.x] = 0u;
+                                                    float* __restrict__ out, float* __restrict__ copy) {
     __shared__ float part[16], pmin[16];
     float m = -INFINITY, mn = INFINITY;
     for (int64_t k = threadIdx.x; k < n; k += 1024) {
@@ -377,26 +373,19 @@ __device__ __forceinline__ void write_dropped(const OutArgs& O, int64_t i) {
 
 // Hit records for the adjoint: the first kHitSlots hits of every photon that
 // reaches the sensor, [kHitSlots][N] float4 (t, b1, b2, obj << 30 | prim), and
-// per photon the number recorded: 0 = dropped (missed the scene or masked by
-// the splat's den != 0 test), kHitsCapped = dropped still bouncing after
-// kMaxBounces hits, kHitsRetrace = more bounces than slots (the adjoint traces
-// that path again).  Neither kind of dropped photon gets a gradient.
+// per photon the number recorded -- or why the photon was dropped: 0 = it
+// missed the scene, kHitsDegenerate = it reached the sensor but the splat's
+// den != 0 test masked it, kHitsCapped = still bouncing after kMaxBounces
+// hits; kHitsRetrace = more bounces than slots (the adjoint traces that path
+// again).  Dropped photons get no gradient.  hf_hits_stats_get counts these
+// codes when asked, so the simulate itself keeps no counters.
 constexpr int kHitSlots = 4;
+constexpr uint8_t kHitsDegenerate = 253;
 constexpr uint8_t kHitsCapped = 254;
 constexpr uint8_t kHitsRetrace = 255;
-static_assert(kMaxBounces < kHitsCapped, "hit counts must stay below the marker values");
+static_assert(kMaxBounces < kHitsDegenerate, "hit counts must stay below the marker values");
 
-// Path outcome counters of one simulate launch (hf_path_stats), one atomic
-// per wavefront and outcome.
 enum { kReached = 0, kMissed = 1, kDegenerate = 2, kCapped = 3 };
-__device__ __forceinline__ void count_outcome(uint32_t* stats, int outcome) {
-    if (!stats) return;
-#pragma unroll
-    for (int k = 0; k < kStatWords; ++k) {
-        const uint64_t m = __ballot(outcome == k);
-        if ((threadIdx.x & 63) == 0 && m) atomicAdd(&stats[k], (uint32_t)__popcll(m));
-    }
-}
 
 __device__ __forceinline__ float4 pack_hit(const Hit& h) {
     return make_float4(h.t, h.b1, h.b2, __int_as_float((int)(((uint32_t)h.obj << 30) | (uint32_t)h.prim)));
@@ -409,9 +398,9 @@ __device__ __forceinline__ Hit unpack_hit(float4 v) {
 
 // One photon's path (model/caustics.py:242-291): emit, then per hit trace,
 // advance, transfer and either splat (ground) or refract, for at most
-// kMaxBounces hits.  Returns the outcome (kReached, kMissed, kDegenerate or
-// kCapped); dropped photons get cp = -1.
-__device__ __forceinline__ int simulate_photon(SceneArgs S, const float* __restrict__ hmax, const EmitArgs& E,
+// kMaxBounces hits.  Dropped photons get cp = -1 and, with hit records, the
+// code of the reason.
+__device__ __forceinline__ void simulate_photon(SceneArgs S, const float* __restrict__ hmax, const EmitArgs& E,
                                                const OutArgs& O, float4* __restrict__ hits,
                                                uint8_t* __restrict__ nhit, int64_t i) {
     S.hmax = hmax[0];
@@ -455,7 +444,7 @@ __device__ __forceinline__ int simulate_photon(SceneArgs S, const float* __restr
             O.radius[i] = sp.radius;
             O.cp[i] = ch;
             if (nhit) nhit[i] = nb <= kHitSlots ? (uint8_t)nb : kHitsRetrace;
-            return kReached;
+            return;
         }
         const Refr R = refr_setup(s.d, ns, et);
         refract_apply(R, s.d, s.Dt, s.Dp);
@@ -464,22 +453,46 @@ __device__ __forceinline__ int simulate_photon(SceneArgs S, const float* __restr
         s.o = p;
     }
     write_dropped(O, i);
-    if (nhit) nhit[i] = outcome == kCapped ? kHitsCapped : (uint8_t)0;
-    return outcome;
+    if (nhit) nhit[i] = outcome == kCapped ? kHitsCapped : outcome == kDegenerate ? kHitsDegenerate : (uint8_t)0;
 }
 
-// hits, nhit: null, or the hit records to write; stats: null, or the
-// kStatWords path counters (cleared by hmax_kernel).
+// hits, nhit: null, or the hit records to write.
 // 5 wavefronts per SIMD (<= 102 VGPRs; a few spill): ~1.5 % faster per
 // Landweber step than the 4 that its natural 103 registers allow
 __global__ __launch_bounds__(256, 5) void simulate_kernel(SceneArgs S, const float* __restrict__ hmax, EmitArgs E,
                                                        OutArgs O, float4* __restrict__ hits,
-                                                       uint8_t* __restrict__ nhit, uint32_t* __restrict__ stats) {
+                                                       uint8_t* __restrict__ nhit) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    // whole wavefronts reach the counters (their ballots need every lane)
-    int outcome = -1;
-    if (i < O.N) outcome = simulate_photon(S, hmax, E, O, hits, nhit, i);
-    count_outcome(stats, outcome);
+    if (i < O.N) simulate_photon(S, hmax, E, O, hits, nhit, i);
+}
+
+// Path outcomes from the hit-count bytes: per-workgroup LDS counters, one
+// device atomic per workgroup and outcome into stats[kStatWords] (cleared by
+// stats_zero_kernel before).
+__global__ __launch_bounds__(256) void stats_zero_kernel(uint32_t* __restrict__ stats) {
+    if (threadIdx.x < kStatWords) stats[threadIdx.x] = 0u;
+}
+
+__global__ __launch_bounds__(256) void path_stats_kernel(const uint8_t* __restrict__ nhit, int64_t N,
+                                                         uint32_t* __restrict__ stats) {
+    __shared__ uint32_t c[kStatWords];
+    if (threadIdx.x < kStatWords) c[threadIdx.x] = 0u;
+    __syncthreads();
+    uint32_t n = 0, miss = 0, deg = 0, cap = 0;
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < N; i += stride) {
+        const uint8_t v = nhit[i];
+        ++n;
+        miss += v == 0 ? 1u : 0u;
+        deg += v == kHitsDegenerate ? 1u : 0u;
+        cap += v == kHitsCapped ? 1u : 0u;
+    }
+    atomicAdd(&c[kReached], n - miss - deg - cap);
+    atomicAdd(&c[kMissed], miss);
+    atomicAdd(&c[kDegenerate], deg);
+    atomicAdd(&c[kCapped], cap);
+    __syncthreads();
+    if (threadIdx.x < kStatWords && c[threadIdx.x]) atomicAdd(&stats[threadIdx.x], c[threadIdx.x]);
 }
 
 // ---------------------------------------------------------------------------
@@ -936,13 +949,11 @@ __global__ __launch_bounds__(256) void vcons_kernel(const float* __restrict__ x,
 unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 // hmax[0] = max and hmax[1] = min of the n heights (hmax: the workspace's
-// hmax region); also written to copy[0..1] when copy is not null; stats (if
-// not null) cleared
-void launch_hmax(const float* h, int64_t n, float* hmax, hipStream_t s, float* copy = nullptr,
-                 uint32_t* stats = nullptr) {
+// hmax region); also written to copy[0..1] when copy is not null
+void launch_hmax(const float* h, int64_t n, float* hmax, hipStream_t s, float* copy = nullptr) {
     hipLaunchKernelGGL(hmax_part_kernel, dim3(kHmaxBlocks), dim3(256), 0, s, h, n, hmax + 64);
     hipLaunchKernelGGL(hmax_kernel, dim3(1), dim3(1024), 0, s, (const float*)(hmax + 64), (int64_t)kHmaxBlocks, hmax,
-                       copy, stats);
+                       copy);
 }
 
 EmitArgs emit_args(const hf_scene* sc, const hf_emitter* em) {
@@ -1004,8 +1015,8 @@ int hip_status(const char* what) {
 
 // hit buffer: records [kHitSlots][N] float4, counts [N] bytes, the maximum
 // and minimum height of the field the records were traced on (the adjoint's
-// re-traced paths use them, so they are not computed again), and the path
-// counters of the simulate that wrote it (hf_path_stats)
+// re-traced paths use them, so they are not computed again), and room for
+// the path counters that hf_hits_stats_get counts
 struct HitsLayout {
     size_t recs, count, hmax, stats, end;
 };
@@ -1039,11 +1050,10 @@ int simulate_impl(const float* h, const float* normals, const hf_scene* sc, cons
     OutArgs O{Ep, xp, Mp, radius, cp, N};
     float4* hrec = want_hits ? (float4*)((char*)hits + HL.recs) : nullptr;
     uint8_t* hcnt = want_hits ? (uint8_t*)hits + HL.count : nullptr;
-    uint32_t* hst = want_hits ? (uint32_t*)((char*)hits + HL.stats) : nullptr;
     hipStream_t s = (hipStream_t)stream;
-    launch_hmax(h, (int64_t)sc->R * sc->R, hmax, s, want_hits ? (float*)((char*)hits + HL.hmax) : nullptr, hst);
+    launch_hmax(h, (int64_t)sc->R * sc->R, hmax, s, want_hits ? (float*)((char*)hits + HL.hmax) : nullptr);
     hipLaunchKernelGGL(simulate_kernel, dim3(blocks_for(N)), dim3(256), 0, s, scene_args(h, normals, sc, hmax),
-                       (const float*)hmax, emit_args(sc, em), O, hrec, hcnt, hst);
+                       (const float*)hmax, emit_args(sc, em), O, hrec, hcnt);
     return hip_status("hf_simulate");
 }
 
@@ -1150,18 +1160,22 @@ int hf_simulate_hits_f32(const float* h, const float* normals, const hf_scene* s
     return simulate_impl(h, normals, sc, em, Ep, xp, Mp, cp, radius, ws, ws_bytes, hits, hits_bytes, true, stream);
 }
 
-int hf_hits_stats_get(const void* hits, size_t hits_bytes, const hf_scene* sc, const hf_emitter* em,
-                      hf_path_stats* out, void* stream) {
+int hf_hits_stats_get(void* hits, size_t hits_bytes, const hf_scene* sc, const hf_emitter* em, hf_path_stats* out,
+                      void* stream) {
     g_err[0] = '\0';
     if (!out) return fail(PDS_ERR_ARGUMENT, "null stats pointer");
     const size_t need = hf_hits_bytes(sc, em);
     if (need == 0) return fail(PDS_ERR_ARGUMENT, "invalid scene or emitter");
     if (!hits || hits_bytes < need) return fail(PDS_ERR_WORKSPACE, "hit buffer smaller than hf_hits_bytes");
     const int64_t N = (int64_t)em->n_side * em->n_side * sc->channels;
+    const HitsLayout HL = hits_layout(N);
+    uint32_t* st = (uint32_t*)((char*)hits + HL.stats);
     uint32_t c[kStatWords];
     hipStream_t s = (hipStream_t)stream;
-    if (hipMemcpyAsync(c, (const char*)hits + hits_layout(N).stats, sizeof(c), hipMemcpyDeviceToHost, s) !=
-            hipSuccess ||
+    hipLaunchKernelGGL(stats_zero_kernel, dim3(1), dim3(256), 0, s, st);
+    hipLaunchKernelGGL(path_stats_kernel, dim3(256), dim3(256), 0, s, (const uint8_t*)hits + HL.count, N, st);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(c, st, sizeof(c), hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
         return hip_status("hf_hits_stats_get");
     out->reached = c[kReached];

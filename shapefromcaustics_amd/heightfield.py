@@ -219,14 +219,15 @@ def trace_rays(h: torch.Tensor, scene: Scene, origins: torch.Tensor, dirs: torch
     return t, uv, obj, prim
 
 
-HITS_CAPPED, HITS_RETRACE = 254, 255
+HITS_DEGENERATE, HITS_CAPPED, HITS_RETRACE = 253, 254, 255
 
 
 def hit_counts(hits: torch.Tensor, n_photons: int) -> torch.Tensor:
     """The per-photon hit counts inside a hit buffer from ``simulate(...,
-    keep_hits=True)``: 0 = dropped (missed the scene or masked by the
-    splat's den != 0 test), 1..4 = hits recorded, 254 (``HITS_CAPPED``) =
-    dropped still bouncing after ``MAX_BOUNCES`` hits, 255
+    keep_hits=True)``: 0 = dropped, it missed the scene; 1..4 = hits
+    recorded; 253 (``HITS_DEGENERATE``) = dropped by the splat's den != 0
+    test; 254 (``HITS_CAPPED``) = dropped still bouncing after
+    ``MAX_BOUNCES`` hits; 255
     (``HITS_RETRACE``) = more bounces than recorded (the adjoint traces that
     path again).  Layout of the buffer (csrc/hf_kernels.hip, hits_layout):
     4 x N float4 records, then the N count bytes at the next 256-byte
