@@ -41,7 +41,8 @@ namespace pds {
 
 namespace {
 
-constexpr int kBinThreads = 512;
+constexpr int kCountThreads = 512;    // threads of a binning workgroup: count pass
+constexpr int kScatterThreads = 512;  // ... and scatter pass
 constexpr int kMaxBins = 16384;
 constexpr int kMaxBinBlocks = 1024;
 constexpr int kTileThreads = 512;
@@ -181,7 +182,6 @@ struct BinPhoton {
     int32_t x0, x1, y0, y1;  // box on the image, inclusive
     Frame f;
     float e;
-    int32_t rx, ry, ox, oy;  // window radii and pxc - ix + 1, pyc - iy + 1
 };
 
 // Photon i's footprint box: window (photon_differentials.cu:80-96) n image n
@@ -254,23 +254,48 @@ __device__ __forceinline__ BinPhoton bin_photon(const RawPhoton& R, const ImageA
     b.y1 = y1;
     b.c = (int32_t)c;
     b.e = e + poison;
-    b.rx = win.rx;
-    b.ry = win.ry;
-    b.ox = win.pxc - b.f.ix + 1;  // in {0, 1, 2}: trunc vs round of the same double
-    b.oy = win.pyc - b.f.iy + 1;
     b.rec = true;
     return b;
 }
 
 // ---------------------------------------------------------------------------
-// 32-byte record: (fx, fy, e, packed), (a00, a01, a10, a11)
-// packed: ix - tile centre (int8) | iy - tile centre (int8) << 8 | rx << 16 |
-//         ry << 22 | ox << 28 | oy << 30
+// 32-byte record of one (photon, tile) overlap, with the photon's footprint
+// box already clipped to the tile (the scatter has the box; the tile kernels
+// then neither recompute the window nor clip to the ellipse):
+//   x  fx quantised to 2^-23 px (signed, bits 8-31) | box x origin in the tile (0-5)
+//   y  fy likewise                                  | box y origin in the tile (0-5)
+//   z  energy (float bits)
+//   w  box width - 1 (0-5) | box height - 1 (6-11) | box x origin - ix + 64 (12-19) |
+//      box y origin - iy + 64 (20-27)
+//   second half: a00, a01, a10, a11 (float bits)
+// fx, fy lie in [-1/2, 1/2], so the quantised fraction is an exact float and
+// within 6e-8 px of the double fraction; box origins relative to (ix, iy)
+// lie in [-64, 64] (window radii <= 63).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t pack_rec(int32_t dcx, int32_t dcy, int32_t rx, int32_t ry, int32_t ox, int32_t oy) {
-    return ((uint32_t)(dcx & 0xff)) | ((uint32_t)(dcy & 0xff) << 8) | ((uint32_t)rx << 16) | ((uint32_t)ry << 22) |
-           ((uint32_t)ox << 28) | ((uint32_t)oy << 30);
+constexpr float kFracScale = 8388608.0f;         // 2^23
+constexpr float kFracUnit = 1.0f / 8388608.0f;   // 2^-23
+
+__device__ __forceinline__ uint32_t quantise_frac(float f) { return (uint32_t)__float2int_rn(f * kFracScale); }
+
+__device__ __forceinline__ float frac_of(uint32_t word) { return (float)((int32_t)word >> 8) * kFracUnit; }
+
+// Record words x, y, w for the photon box [x0, x1] x [y0, y1] (already within
+// the image) in the tile with origin (X0, Y0), edge T; fxq, fyq from quantise_frac.
+__device__ __forceinline__ uint4 rec_head(uint32_t fxq, uint32_t fyq, float e, int32_t ix, int32_t iy, int32_t x0,
+                                          int32_t x1, int32_t y0, int32_t y1, int32_t X0, int32_t Y0, int32_t T) {
+    const int32_t bx0 = max(x0, X0), bx1 = min(x1, X0 + T - 1);
+    const int32_t by0 = max(y0, Y0), by1 = min(y1, Y0 + T - 1);
+    uint4 q;
+    q.x = (fxq << 8) | (uint32_t)(bx0 - X0);
+    q.y = (fyq << 8) | (uint32_t)(by0 - Y0);
+    q.z = __float_as_uint(e);
+    q.w = (uint32_t)(bx1 - bx0) | ((uint32_t)(by1 - by0) << 6) | ((uint32_t)(bx0 - ix + 64) << 12) |
+          ((uint32_t)(by0 - iy + 64) << 20);
+    return q;
 }
+
+// the box walk's length (area) of a record, from its w word
+__device__ __forceinline__ int rec_area(uint32_t w) { return (int)((w & 63) + 1) * (int)(((w >> 6) & 63) + 1); }
 
 // ---------------------------------------------------------------------------
 // 0. init: totals and the control block (no memset, so the sequence can be
@@ -295,13 +320,13 @@ __global__ __launch_bounds__(1024) void bin_init_kernel(Geom g, uint32_t* __rest
 //    fixed-point sums are exact, and every record's gradient partial is
 //    computed on its own).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBinThreads) void bin_count_kernel(PhotonArgs P, ImageArgs I, Geom g,
+__global__ __launch_bounds__(kCountThreads) void bin_count_kernel(PhotonArgs P, ImageArgs I, Geom g,
                                                                 uint32_t* __restrict__ hist,
                                                                 uint32_t* __restrict__ totals,
                                                                 float* __restrict__ emax_blk) {
     extern __shared__ uint32_t lh[];  // nbins entries
-    __shared__ float wmax[kBinThreads / 64];
-    for (int k = threadIdx.x; k < g.nbins; k += kBinThreads) lh[k] = 0;
+    __shared__ float wmax[kCountThreads / 64];
+    for (int k = threadIdx.x; k < g.nbins; k += kCountThreads) lh[k] = 0;
     __syncthreads();
     const int64_t lo = (int64_t)blockIdx.x * g.per_block;
     const int64_t hi = min(lo + g.per_block, live_photons(P));
@@ -310,11 +335,11 @@ __global__ __launch_bounds__(kBinThreads) void bin_count_kernel(PhotonArgs P, Im
     // two photons of this lane in flight while one is binned
     RawPhoton r0 = {}, r1 = {};
     if (i < hi) r0 = load_photon(P, i, g.det, false);
-    if (i + kBinThreads < hi) r1 = load_photon(P, i + kBinThreads, g.det, false);
-    for (; i < hi; i += kBinThreads) {
+    if (i + kCountThreads < hi) r1 = load_photon(P, i + kCountThreads, g.det, false);
+    for (; i < hi; i += kCountThreads) {
         const RawPhoton now = r0;
         r0 = r1;
-        if (i + 2 * kBinThreads < hi) r1 = load_photon(P, i + 2 * kBinThreads, g.det, false);
+        if (i + 2 * kCountThreads < hi) r1 = load_photon(P, i + 2 * kCountThreads, g.det, false);
         const BinPhoton b = bin_photon(now, I);
         if (!b.rec) continue;
         if (b.e > 0.0f && isfinite(b.e)) emax = fmaxf(emax, b.e);
@@ -329,13 +354,13 @@ __global__ __launch_bounds__(kBinThreads) void bin_count_kernel(PhotonArgs P, Im
     }
     __syncthreads();
     uint32_t* row = hist + (size_t)blockIdx.x * g.nbins;
-    for (int k = threadIdx.x; k < g.nbins; k += kBinThreads) {
+    for (int k = threadIdx.x; k < g.nbins; k += kCountThreads) {
         const uint32_t n = lh[k];
         if (n) row[k] = atomicAdd(&totals[k], n);  // entries of empty bins are never read
     }
     if (g.det && threadIdx.x == 0) {
         float m = 0.0f;
-        for (int w = 0; w < kBinThreads / 64; ++w) m = fmaxf(m, wmax[w]);
+        for (int w = 0; w < kCountThreads / 64; ++w) m = fmaxf(m, wmax[w]);
         emax_blk[blockIdx.x] = m;
     }
 }
@@ -436,7 +461,7 @@ __global__ __launch_bounds__(1024) void bin_plan_kernel(Geom g, int64_t N, const
 //    bins, which the tile kernels then accumulate into atomically.
 // ---------------------------------------------------------------------------
 template <bool KEEP>
-__global__ __launch_bounds__(kBinThreads) void bin_scatter_kernel(PhotonArgs P, ImageArgs I, Geom g,
+__global__ __launch_bounds__(kScatterThreads) void bin_scatter_kernel(PhotonArgs P, ImageArgs I, Geom g,
                                                                   const uint32_t* __restrict__ hist,
                                                                   const uint32_t* __restrict__ binoff,
                                                                   const int4* __restrict__ segs,
@@ -450,8 +475,8 @@ __global__ __launch_bounds__(kBinThreads) void bin_scatter_kernel(PhotonArgs P, 
     if (ctrl->overflow) {
         if (!clear) return;
         const int64_t n = (int64_t)I.C * I.H * I.W;
-        const int64_t stride = (int64_t)gridDim.x * kBinThreads;
-        for (int64_t k = (int64_t)blockIdx.x * kBinThreads + threadIdx.x; k < n; k += stride) {
+        const int64_t stride = (int64_t)gridDim.x * kScatterThreads;
+        for (int64_t k = (int64_t)blockIdx.x * kScatterThreads + threadIdx.x; k < n; k += stride) {
             if (g.det)
                 acc64[k] = 0ull;
             else
@@ -460,29 +485,28 @@ __global__ __launch_bounds__(kBinThreads) void bin_scatter_kernel(PhotonArgs P, 
         return;
     }
     const uint32_t* row = hist + (size_t)blockIdx.x * g.nbins;
-    for (int k = threadIdx.x; k < g.nbins; k += kBinThreads) cur[k] = binoff[k] + row[k];
+    for (int k = threadIdx.x; k < g.nbins; k += kScatterThreads) cur[k] = binoff[k] + row[k];
     __syncthreads();
     const int64_t lo = (int64_t)blockIdx.x * g.per_block;
     const int64_t hi = min(lo + g.per_block, live_photons(P));
-    const int half = g.T >> 1;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     // Each round of records is staged in LDS and written by lane pairs, one
     // 32-byte record per pair (lane 2s: first half, lane 2s+1: second half of
     // lane s's record), so that the two halves of a record leave in one
     // contiguous request instead of two scattered 16-byte ones.
-    __shared__ uint4 stage[kBinThreads / 64][2][64];
-    __shared__ uint32_t stage_slot[kBinThreads / 64][64];
+    __shared__ uint4 stage[kScatterThreads / 64][2][64];
+    __shared__ uint32_t stage_slot[kScatterThreads / 64][64];
     int64_t i = lo + threadIdx.x;
     // two photons of this lane in flight while one is binned
     RawPhoton r0 = {}, r1 = {};
     if (i < hi) r0 = load_photon(P, i);
-    if (i + kBinThreads < hi) r1 = load_photon(P, i + kBinThreads);
+    if (i + kScatterThreads < hi) r1 = load_photon(P, i + kScatterThreads);
     // wavefront-uniform rounds (the pair writes need the whole wavefront)
-    for (int64_t first = lo + (threadIdx.x & ~63); first < hi; first += kBinThreads, i += kBinThreads) {
+    for (int64_t first = lo + (threadIdx.x & ~63); first < hi; first += kScatterThreads, i += kScatterThreads) {
         const bool valid = i < hi;
         const RawPhoton now = r0;
         r0 = r1;
-        if (i + 2 * kBinThreads < hi) r1 = load_photon(P, i + 2 * kBinThreads);
+        if (i + 2 * kScatterThreads < hi) r1 = load_photon(P, i + 2 * kScatterThreads);
         BinPhoton b = {};
         if (valid) b = bin_photon(now, I);
         int tx0 = 0, ty0 = 0, nx = 1, nrec = 0;
@@ -494,7 +518,7 @@ __global__ __launch_bounds__(kBinThreads) void bin_scatter_kernel(PhotonArgs P, 
         }
         int kmax = nrec;
         for (int o = 32; o > 0; o >>= 1) kmax = max(kmax, __shfl_xor(kmax, o));
-        const uint4 h0 = make_uint4(__float_as_uint(b.f.fx), __float_as_uint(b.f.fy), __float_as_uint(b.e), 0u);
+        const uint32_t fxq = quantise_frac(b.f.fx), fyq = quantise_frac(b.f.fy);
         const uint4 h1 = make_uint4(__float_as_uint(b.f.a00), __float_as_uint(b.f.a01), __float_as_uint(b.f.a10),
                                     __float_as_uint(b.f.a11));
         int kx = 0, ky = 0;  // this lane's current tile, relative to (tx0, ty0), row-major
@@ -503,10 +527,8 @@ __global__ __launch_bounds__(kBinThreads) void bin_scatter_kernel(PhotonArgs P, 
             if (has) {
                 const int tx = tx0 + kx, ty = ty0 + ky;
                 const uint32_t slot = atomicAdd(&cur[(b.c * g.tiles_y + ty) * g.tiles_x + tx], 1u);
-                uint4 q = h0;
-                q.w = pack_rec(b.f.ix - ((tx << g.tshift) + half), b.f.iy - ((ty << g.tshift) + half), b.rx, b.ry,
-                               b.ox, b.oy);
-                stage[w][0][lane] = q;
+                stage[w][0][lane] = rec_head(fxq, fyq, b.e, b.f.ix, b.f.iy, b.x0, b.x1, b.y0, b.y1, tx << g.tshift,
+                                             ty << g.tshift, g.T);
                 stage[w][1][lane] = h1;
                 stage_slot[w][lane] = slot;
                 if (KEEP && k < kMaxRecsPerPhoton) slots[(size_t)k * P.N + i] = slot;
@@ -536,7 +558,7 @@ __global__ __launch_bounds__(kBinThreads) void bin_scatter_kernel(PhotonArgs P, 
         const int c = d.x / tpc, t = d.x - c * tpc;
         const int ty = t / g.tiles_x, tx = t - ty * g.tiles_x;
         const int X0 = tx << g.tshift, Y0 = ty << g.tshift;
-        for (int k = threadIdx.x; k < g.T * g.T; k += kBinThreads) {
+        for (int k = threadIdx.x; k < g.T * g.T; k += kScatterThreads) {
             const int px = X0 + (k & (g.T - 1)), py = Y0 + (k >> g.tshift);
             if (px < I.W && py < I.H) {
                 const int64_t o = ((int64_t)c * I.H + py) * I.W + px;
@@ -694,45 +716,6 @@ struct BoxWalk {
     }
 };
 
-// The record's frame and energy for the tile with origin (X0, Y0); also the
-// reference window (radii rx, ry about pixel (pxc, pyc)).
-__device__ __forceinline__ void decode_frame_q(const uint4& q0, const uint4& q1, int X0, int Y0, int half, TileRec& t,
-                                               int& rx, int& ry, int& pxc, int& pyc) {
-    const uint32_t pk = q0.w;
-    const int ix = X0 + half + (int)(int8_t)(pk & 0xff);
-    const int iy = Y0 + half + (int)(int8_t)((pk >> 8) & 0xff);
-    rx = (pk >> 16) & 63;
-    ry = (pk >> 22) & 63;
-    pxc = ix + (int)((pk >> 28) & 3) - 1;
-    pyc = iy + (int)(pk >> 30) - 1;
-    t.f.ix = ix;
-    t.f.iy = iy;
-    t.f.fx = __uint_as_float(q0.x);
-    t.f.fy = __uint_as_float(q0.y);
-    t.e = __uint_as_float(q0.z);
-    t.f.a00 = __uint_as_float(q1.x);
-    t.f.a01 = __uint_as_float(q1.y);
-    t.f.a10 = __uint_as_float(q1.z);
-    t.f.a11 = __uint_as_float(q1.w);
-}
-
-__device__ __forceinline__ void decode_frame(const uint4* __restrict__ recs, size_t j, int X0, int Y0, int half,
-                                             TileRec& t, int& rx, int& ry, int& pxc, int& pyc) {
-    decode_frame_q(recs[2 * j], recs[2 * j + 1], X0, Y0, half, t, rx, ry, pxc, pyc);
-}
-
-// Box fields from the box's first pixel (x0, y0), width and height (rows = 0:
-// empty box, which no walk visits).
-__device__ __forceinline__ void set_tile_box(TileRec& t, int x0, int y0, int bw, int rows, int X0, int Y0) {
-    t.bw = bw;
-    t.rows = rows;
-    t.area = bw * rows;
-    t.lx0 = x0 - X0;
-    t.ly0 = y0 - Y0;
-    t.kx0 = x0 - t.f.ix;
-    t.ky0 = y0 - t.f.iy;
-}
-
 // Chord parameters (the chord walk only).
 __device__ __forceinline__ void chord_setup(TileRec& t, bool box_walk) {
     t.full = true;  // the box walk visits whole box rows: no chord parameters
@@ -748,84 +731,95 @@ __device__ __forceinline__ void chord_setup(TileRec& t, bool box_walk) {
     }
 }
 
-__device__ __forceinline__ TileRec decode_tile_rec_q(const uint4& q0, const uint4& q1, int X0, int Y0, int X1, int Y1,
-                                                     int half, bool box_walk) {
+// One record decoded for the tile with origin (X0, Y0): its frame, energy and
+// footprint box (window n tile n ellipse bounding box, made by the scatter).
+__device__ __forceinline__ TileRec decode_tile_rec_q(const uint4& q0, const uint4& q1, int X0, int Y0,
+                                                     bool box_walk) {
     TileRec t;
-    int rx, ry, pxc, pyc;
-    decode_frame_q(q0, q1, X0, Y0, half, t, rx, ry, pxc, pyc);
-    int x0 = max(max(pxc - rx, X0), 0), x1 = min(pxc + rx, X1);
-    int y0 = max(max(pyc - ry, Y0), 0), y1 = min(pyc + ry, Y1);
-    clip_to_ellipse(t.f, x0, x1, y0, y1);
-    const bool any = x0 <= x1 && y0 <= y1;
-    set_tile_box(t, x0, y0, any ? x1 - x0 + 1 : 1, any ? y1 - y0 + 1 : 0, X0, Y0);
+    const uint32_t w = q0.w;
+    t.lx0 = (int)(q0.x & 63);
+    t.ly0 = (int)(q0.y & 63);
+    t.bw = (int)(w & 63) + 1;
+    t.rows = (int)((w >> 6) & 63) + 1;
+    t.area = t.bw * t.rows;
+    t.kx0 = (int)((w >> 12) & 255) - 64;
+    t.ky0 = (int)((w >> 20) & 255) - 64;
+    t.f.ix = X0 + t.lx0 - t.kx0;
+    t.f.iy = Y0 + t.ly0 - t.ky0;
+    t.f.fx = frac_of(q0.x);
+    t.f.fy = frac_of(q0.y);
+    t.e = __uint_as_float(q0.z);
+    t.f.a00 = __uint_as_float(q1.x);
+    t.f.a01 = __uint_as_float(q1.y);
+    t.f.a10 = __uint_as_float(q1.z);
+    t.f.a11 = __uint_as_float(q1.w);
     chord_setup(t, box_walk);
     return t;
 }
 
-__device__ __forceinline__ TileRec decode_tile_rec(const uint4* __restrict__ recs, size_t j, int X0, int Y0, int X1,
-                                                   int Y1, int half, bool box_walk) {
-    return decode_tile_rec_q(recs[2 * j], recs[2 * j + 1], X0, Y0, X1, Y1, half, box_walk);
+__device__ __forceinline__ TileRec decode_tile_rec(const uint4* __restrict__ recs, size_t j, int X0, int Y0,
+                                                   bool box_walk) {
+    return decode_tile_rec_q(recs[2 * j], recs[2 * j + 1], X0, Y0, box_walk);
 }
 
-// A decoded record's walk length (its sort key, clamped to 255) and box in
-// one word: lx0 | ly0 << 6 | (bw - 1) << 12 | (rows - 1) << 18 | key << 24
-// (tile edge <= 64), so that a second pass over the segment's records needs
-// no window or ellipse clipping.  Key 0: nothing to walk (box left out).
-__device__ __forceinline__ uint32_t pack_tile_box(const TileRec& t, uint32_t key) {
-    if (key == 0u) return 0u;
-    return (uint32_t)t.lx0 | ((uint32_t)t.ly0 << 6) | ((uint32_t)(t.bw - 1) << 12) |
-           ((uint32_t)(t.rows - 1) << 18) | (key << 24);
-}
-
-// decode_tile_rec with the box given by pack_tile_box (key > 0): the same record.
-__device__ __forceinline__ TileRec decode_tile_rec_boxed(const uint4* __restrict__ recs, size_t j, int X0, int Y0,
-                                                         int half, uint32_t box, bool box_walk) {
-    TileRec t;
-    int rx, ry, pxc, pyc;
-    decode_frame(recs, j, X0, Y0, half, t, rx, ry, pxc, pyc);
-    const int lx0 = box & 63, ly0 = (box >> 6) & 63, bw = ((box >> 12) & 63) + 1, rows = ((box >> 18) & 63) + 1;
-    set_tile_box(t, X0 + lx0, Y0 + ly0, bw, rows, X0, Y0);
-    chord_setup(t, box_walk);
-    return t;
-}
-
-// Counting sort of a segment's records by the pixels of their chords (clamped
-// to 255; the key is the exact count below that), so that the 64 records a
-// wavefront walks side by side, one per lane, have nearly the same length.
+// Counting sort of a segment's records by their walk length (box area, or
+// the pixels of their chords; clamped to 255, the key is exact below that),
+// so that the 64 records a wavefront walks side by side, one per lane, have
+// nearly the same length.  The box walk's length is in each record's w word:
+// that word is all this pass reads.
 struct SegSort {
-    uint32_t box[kMaxSeg];  // pack_tile_box of every record (key in bits 24-31)
+    uint8_t key[kMaxSeg];
     uint16_t order[kMaxSeg];
     uint32_t hist[kSortKeys];
     uint32_t wsum[kSortKeys / 64];
 };
 
 __device__ __forceinline__ void sort_segment(SegSort& S, const uint4* __restrict__ recs, const int4& d, int X0, int Y0,
-                                             int X1, int Y1, int half, bool box_walk) {
+                                             bool box_walk) {
     static_assert(kTileThreads >= kSortKeys && kSortKeys % 64 == 0, "one histogram bin per thread of the scan");
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     const bool scan = tid < kSortKeys;  // whole wavefronts
     if (scan) S.hist[tid] = 0;
     __syncthreads();
     // at most kMaxSeg / kTileThreads records per thread, all loaded before any
-    // is decoded so that the loads are in flight together
+    // key is computed so that the loads are in flight together
     constexpr int R = kMaxSeg / kTileThreads;
-    uint4 q0[R], q1[R];
+    if (box_walk) {
+        const uint32_t* words = reinterpret_cast<const uint32_t*>(recs);
+        uint32_t ww[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int j = tid + r * kTileThreads;
-        if (j < d.z) {
-            q0[r] = recs[2 * ((size_t)d.y + j)];
-            q1[r] = recs[2 * ((size_t)d.y + j) + 1];
+        for (int r = 0; r < R; ++r) {
+            const int j = tid + r * kTileThreads;
+            ww[r] = j < d.z ? words[8 * ((size_t)d.y + j) + 3] : 0u;
         }
-    }
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int j = tid + r * kTileThreads;
-        if (j < d.z) {
-            const TileRec t = decode_tile_rec_q(q0[r], q1[r], X0, Y0, X1, Y1, half, box_walk);
-            const uint32_t key = (uint32_t)min(chord_pixels(t), 255);
-            S.box[j] = pack_tile_box(t, key);
-            atomicAdd(&S.hist[key], 1u);
+        for (int r = 0; r < R; ++r) {
+            const int j = tid + r * kTileThreads;
+            if (j < d.z) {
+                const uint32_t key = (uint32_t)min(rec_area(ww[r]), 255);
+                S.key[j] = (uint8_t)key;
+                atomicAdd(&S.hist[key], 1u);
+            }
+        }
+    } else {
+        uint4 q0[R], q1[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int j = tid + r * kTileThreads;
+            if (j < d.z) {
+                q0[r] = recs[2 * ((size_t)d.y + j)];
+                q1[r] = recs[2 * ((size_t)d.y + j) + 1];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int j = tid + r * kTileThreads;
+            if (j < d.z) {
+                const TileRec t = decode_tile_rec_q(q0[r], q1[r], X0, Y0, false);
+                const uint32_t key = (uint32_t)min(chord_pixels(t), 255);
+                S.key[j] = (uint8_t)key;
+                atomicAdd(&S.hist[key], 1u);
+            }
         }
     }
     __syncthreads();
@@ -843,7 +837,7 @@ __device__ __forceinline__ void sort_segment(SegSort& S, const uint4* __restrict
     }
     __syncthreads();
     for (int j = tid; j < d.z; j += kTileThreads) {
-        const uint32_t pos = atomicAdd(&S.hist[S.box[j] >> 24], 1u);
+        const uint32_t pos = atomicAdd(&S.hist[S.key[j]], 1u);
         S.order[pos] = (uint16_t)j;
     }
     __syncthreads();
@@ -860,6 +854,31 @@ __device__ __forceinline__ int next_chunk(int* counter, int nchunks) {
     if ((threadIdx.x & 63) == 0) c = atomicAdd(counter, 1);
     c = __shfl(c, 0);
     return c < nchunks ? nchunks - 1 - c : -1;
+}
+
+// One lane's record of a queue chunk: chunk index, walk length key (0:
+// nothing to walk, including lanes past the segment's end) and the record's
+// two halves.
+struct ChunkRecs {
+    int ch, key;
+    uint4 q0, q1;
+};
+
+__device__ __forceinline__ ChunkRecs load_chunk(const SegSort& S, const uint4* __restrict__ recs, const int4& d,
+                                                int ch) {
+    ChunkRecs c;
+    c.ch = ch;
+    c.key = 0;
+    c.q0 = c.q1 = make_uint4(0u, 0u, 0u, 0u);
+    if (ch < 0) return c;
+    const int pos = (ch << 6) + (threadIdx.x & 63);
+    if (pos >= d.z) return c;
+    const int j = S.order[pos];
+    c.key = (int)S.key[j];
+    if (c.key == 0) return c;
+    c.q0 = recs[2 * ((size_t)d.y + j)];
+    c.q1 = recs[2 * ((size_t)d.y + j) + 1];
+    return c;
 }
 
 constexpr int kXcds = 8;  // MI355X: 8 accelerator complex dies, each with its own L2
@@ -958,23 +977,21 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_kernel(Geom g, PhotonAr
     if (threadIdx.x == 0) wq = 0;
     for (int k = threadIdx.x; k < TT; k += kTileThreads) tile[k] = Acc(0);
     const SegTile st = seg_tile(g, I, d.x);
-    const int half = T >> 1;
-    sort_segment(S, recs, d, st.X0, st.Y0, st.X1, st.Y1, half, g.box_walk);
+    sort_segment(S, recs, d, st.X0, st.Y0, g.box_walk);
     if (order_out)  // keep the walk order (record | key << 16) for the backward
         for (int pos = threadIdx.x; pos < d.z; pos += kTileThreads) {
             const uint32_t j = S.order[pos];
-            order_out[(size_t)d.y + pos] = j | ((S.box[j] >> 24) << 16);
+            order_out[(size_t)d.y + pos] = j | ((uint32_t)S.key[j] << 16);
         }
     const int nchunks = (d.z + 63) >> 6;
+    // (loading the next chunk's records while the current one walks measured
+    // slower: 81 against 74 us, the registers they hold cost occupancy)
     for (int ch; (ch = next_chunk(&wq, nchunks)) >= 0;) {
-        const int pos = (ch << 6) + (threadIdx.x & 63);
-        if (pos >= d.z) continue;
-        const int j = S.order[pos];
-        const uint32_t box = S.box[j];
-        const int key = (int)(box >> 24);
+        const ChunkRecs cur = load_chunk(S, recs, d, ch);
+        const int key = cur.key;
         if (key == 0) continue;
-        const TileRec t = decode_tile_rec_boxed(recs, (size_t)d.y + j, st.X0, st.Y0, half, box, g.box_walk);
-        const int n = key < 255 ? key : chord_pixels(t);
+        const TileRec t = decode_tile_rec_q(cur.q0, cur.q1, st.X0, st.Y0, g.box_walk);
+        const int n = key < 255 ? key : (g.box_walk ? t.area : chord_pixels(t));
         auto walk = [&](auto& w) {
             for (int p = 0; p < n; ++p) {
                 const float dx = w.dx(t.f.fx);
@@ -1089,16 +1106,15 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_bwd_kernel(Geom g, Imag
         const int px = st.X0 + (k & (T - 1)), py = st.Y0 + (k >> g.tshift);
         gt[k] = (px < I.W && py < I.H) ? grad[((int64_t)st.c * I.H + py) * I.W + px] : 0.0f;
     }
-    const int half = T >> 1;
     const uint16_t* order = nullptr;
-    const uint32_t* boxes = nullptr;
+    const uint8_t* keys = nullptr;
     __shared__ int wq;  // next_chunk's counter
     if (threadIdx.x == 0) wq = 0;
     if constexpr (!SORTED) {
         __shared__ SegSort S;
-        sort_segment(S, recs, d, st.X0, st.Y0, st.X1, st.Y1, half, g.box_walk);
+        sort_segment(S, recs, d, st.X0, st.Y0, g.box_walk);
         order = S.order;
-        boxes = S.box;
+        keys = S.key;
     } else {
         __syncthreads();
     }
@@ -1120,11 +1136,11 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_bwd_kernel(Geom g, Imag
                 key = (int)(w >> 16);
             } else {
                 j = order[pos];
-                key = (int)(boxes[j] >> 24);
+                key = (int)keys[j];
             }
             const size_t slot = (size_t)d.y + j;
-            const TileRec t = decode_tile_rec(recs, slot, st.X0, st.Y0, st.X1, st.Y1, half, g.box_walk);
-            const int n = key < 255 ? key : chord_pixels(t);
+            const TileRec t = decode_tile_rec(recs, slot, st.X0, st.Y0, g.box_walk);
+            const int n = key < 255 ? key : (g.box_walk ? t.area : chord_pixels(t));
             // Per pixel, with w = (1 - q) inside the footprint and 0 outside,
             // accumulate  m0 = sum w^2 g  and the moments of  w g  over the pixel
             // offsets (dx, dy):  m1 = sum w g dx, m2 = sum w g dy, m3 = sum w g dx^2,
@@ -1386,16 +1402,16 @@ void launch_build(const PhotonArgs& P, const ImageArgs& I, const Geom& g, const 
                   hipStream_t s) {
     const size_t hist_lds = sizeof(uint32_t) * (size_t)g.nbins;
     hipLaunchKernelGGL(bin_init_kernel, dim3(1), dim3(1024), 0, s, g, B.totals, B.ctrl);
-    hipLaunchKernelGGL(bin_count_kernel, dim3(g.G), dim3(kBinThreads), hist_lds, s, P, I, g, B.hist, B.totals,
+    hipLaunchKernelGGL(bin_count_kernel, dim3(g.G), dim3(kCountThreads), hist_lds, s, P, I, g, B.hist, B.totals,
                        B.emax);
     hipLaunchKernelGGL(bin_plan_kernel, dim3(1), dim3(1024), 0, s, g, P.N, (const uint32_t*)B.totals,
                        (const float*)B.emax, B.binoff, B.segs, B.ctrl);
     if (keep)
-        hipLaunchKernelGGL(bin_scatter_kernel<true>, dim3(g.G), dim3(kBinThreads), hist_lds, s, P, I, g,
+        hipLaunchKernelGGL(bin_scatter_kernel<true>, dim3(g.G), dim3(kScatterThreads), hist_lds, s, P, I, g,
                            (const uint32_t*)B.hist, (const uint32_t*)B.binoff, (const int4*)B.segs, B.recs, B.slots,
                            B.info, (const Ctrl*)B.ctrl, out, B.acc64);
     else
-        hipLaunchKernelGGL(bin_scatter_kernel<false>, dim3(g.G), dim3(kBinThreads), hist_lds, s, P, I, g,
+        hipLaunchKernelGGL(bin_scatter_kernel<false>, dim3(g.G), dim3(kScatterThreads), hist_lds, s, P, I, g,
                            (const uint32_t*)B.hist, (const uint32_t*)B.binoff, (const int4*)B.segs, B.recs,
                            (uint32_t*)nullptr, (uint8_t*)nullptr, (const Ctrl*)B.ctrl, out, B.acc64);
 }
