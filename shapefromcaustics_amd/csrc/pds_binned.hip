@@ -294,8 +294,10 @@ __device__ __forceinline__ uint4 rec_head(uint32_t fxq, uint32_t fyq, float e, i
     return q;
 }
 
-// the box walk's length (area) of a record, from its w word
-__device__ __forceinline__ int rec_area(uint32_t w) { return (int)((w & 63) + 1) * (int)(((w >> 6) & 63) + 1); }
+// the box walk's length of a record, from its w word: box rows x pixel pairs per row
+__device__ __forceinline__ int rec_pairs(uint32_t w) {
+    return (int)(((w & 63) + 2) >> 1) * (int)(((w >> 6) & 63) + 1);
+}
 
 // ---------------------------------------------------------------------------
 // 0. init: totals and the control block (no memset, so the sequence can be
@@ -685,34 +687,67 @@ struct ChordWalk {
     }
 };
 
-// The walk over a record's whole box rows (the default): the same pixels,
-// order, dx, dy and LDS index as ChordWalk over a full box, with a
-// branch-free row advance (a divergent seek would cost every lane of the
-// wavefront whenever one lane changes row, which is most steps).
+// The walk over a record's whole box rows (the default), two pixels of a row
+// per step: (kx, ky) and (kx + 1, ky), the second absent (second() false)
+// where a row of odd width ends.  A pair shares its row's products a01 dy and
+// a11 dy and pays the branch-free row advance once (a divergent advance would
+// cost every lane of the wavefront whenever one lane changes row, which is
+// most steps).  Per pixel the arithmetic is ChordWalk's (the same dx, dy, u,
+// v, q), and the pixels come in the same order.
 //
 // The pixel offsets (kx, ky) are counted in float: they are small integers,
 // exact in float, so (kx - fx) is the same float as ((float)kx - fx) and the
 // walk spends no int-to-float conversions per pixel.
-struct BoxWalk {
-    float kx, ky, kx0, xb;
-    int li, skip;
-    float dy;
-    __device__ __forceinline__ float dx(float fx) const { return kx - fx; }
+struct PairWalk {
+    float kx, ky, kx0, xb, xb1;  // xb: last column; a pair at kx >= xb1 = xb - 1 ends its row
+    int li, skip;                // LDS index of the pair's first pixel; its step to the next row
     __device__ __forceinline__ void start(const TileRec& t, int tshift) {
         kx0 = (float)t.kx0;
         kx = kx0;
         ky = (float)t.ky0;
         xb = (float)(t.kx0 + t.bw - 1);
+        xb1 = xb - 1.0f;
         li = (t.ly0 << tshift) + t.lx0;
-        skip = (1 << tshift) - t.bw + 1;
-        dy = ky - t.f.fy;
+        skip = (1 << tshift) - 2 * (((t.bw + 1) >> 1) - 1);
     }
-    __device__ __forceinline__ void step(const TileRec& t, int, bool) {
-        const bool nl = kx >= xb;
-        kx = nl ? kx0 : kx + 1.0f;
-        li += nl ? skip : 1;
+    __device__ __forceinline__ bool second() const { return kx < xb; }
+    __device__ __forceinline__ void step() {
+        const bool nl = kx >= xb1;
+        kx = nl ? kx0 : kx + 2.0f;
+        li += nl ? skip : 2;
         ky += nl ? 1.0f : 0.0f;
-        dy = ky - t.f.fy;
+    }
+};
+
+// steps of the pair walk: box rows x pairs per row
+__device__ __forceinline__ int walk_pairs(const TileRec& t) { return t.rows * ((t.bw + 1) >> 1); }
+
+// The forward's value at a pixel, with the Silverman kernel's 3/pi folded into
+// e3 = (3/pi) E: (1 - q)^2 e3 for q < 1, else 0 (also for a NaN q: fmaxf
+// returns its number operand).  As in the reference (photon_differentials.cu:
+// 105-106) only val > 0 is splatted, so E <= 0 and NaN energies add nothing.
+__device__ __forceinline__ float splat_val(float q, float e3) {
+    const float t = fmaxf(1.0f - q, 0.0f);
+    return t * t * e3;
+}
+
+// The backward's per-pixel footprint weight w = 1 - q inside, 0 outside (and
+// for a NaN q), and the moments of one record's walk:  m0 = sum w^2 g,
+// m1 = sum w g dx, m2 = sum w g dy, m3 = sum w g dx^2, m4 = sum w g dx dy,
+// m5 = sum w g dy^2.
+__device__ __forceinline__ float foot_w(float q) { return fmaxf(1.0f - q, 0.0f); }
+
+struct Moments {
+    float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f, m4 = 0.f, m5 = 0.f;
+    // a pixel of weight w and weighted gradient wg = w g (0 adds exactly nothing)
+    __device__ __forceinline__ void add(float w, float wg, float dx, float dy) {
+        m0 = fmaf(w, wg, m0);
+        const float px = wg * dx, py = wg * dy;
+        m1 += px;
+        m2 += py;
+        m3 = fmaf(px, dx, m3);
+        m4 = fmaf(px, dy, m4);
+        m5 = fmaf(py, dy, m5);
     }
 };
 
@@ -796,7 +831,7 @@ __device__ __forceinline__ void sort_segment(SegSort& S, const uint4* __restrict
         for (int r = 0; r < R; ++r) {
             const int j = tid + r * kTileThreads;
             if (j < d.z) {
-                const uint32_t key = (uint32_t)min(rec_area(ww[r]), 255);
+                const uint32_t key = (uint32_t)min(rec_pairs(ww[r]), 255);
                 S.key[j] = (uint8_t)key;
                 atomicAdd(&S.hist[key], 1u);
             }
@@ -991,34 +1026,44 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_kernel(Geom g, PhotonAr
         const int key = cur.key;
         if (key == 0) continue;
         const TileRec t = decode_tile_rec_q(cur.q0, cur.q1, st.X0, st.Y0, g.box_walk);
-        const int n = key < 255 ? key : (g.box_walk ? t.area : chord_pixels(t));
-        auto walk = [&](auto& w) {
-            for (int p = 0; p < n; ++p) {
-                const float dx = w.dx(t.f.fx);
-                float u, v;
-                const float val = silverman(eval_q(t.f, dx, w.dy, u, v)) * t.e;
-                if (val > 0.0f) {
-                    if constexpr (DET) {
-                        const unsigned long long fxv = to_fixed(val, fe);
-                        if (fxv >> 62)
-                            atomicOr(&tile[w.li], fxv);
-                        else
-                            atomicAdd(&tile[w.li], fxv);
-                    } else {
-                        atomicAdd(&tile[w.li], (double)val);
-                    }
+        const int n = key < 255 ? key : (g.box_walk ? walk_pairs(t) : chord_pixels(t));
+        const float e3 = kThreeOverPi * t.e;
+        auto deposit = [&](int li, float val) {
+            if (val > 0.0f) {
+                if constexpr (DET) {
+                    const unsigned long long fxv = to_fixed(val, fe);
+                    if (fxv >> 62)
+                        atomicOr(&tile[li], fxv);
+                    else
+                        atomicAdd(&tile[li], fxv);
+                } else {
+                    atomicAdd(&tile[li], (double)val);
                 }
-                w.step(t, g.tshift, p + 1 < n);
             }
         };
         if (g.box_walk) {
-            BoxWalk w;
+            PairWalk w;
             w.start(t, g.tshift);
-            walk(w);
+            for (int p = 0; p < n; ++p) {
+                const float dy = w.ky - t.f.fy;
+                const float ru = t.f.a01 * dy, rv = t.f.a11 * dy;
+                const float dx0 = w.kx - t.f.fx, dx1 = (w.kx + 1.0f) - t.f.fx;
+                const float u0 = fmaf(t.f.a00, dx0, ru), v0 = fmaf(t.f.a10, dx0, rv);
+                const float u1 = fmaf(t.f.a00, dx1, ru), v1 = fmaf(t.f.a10, dx1, rv);
+                const float val0 = splat_val(fmaf(u0, u0, v0 * v0), e3);
+                const float val1 = w.second() ? splat_val(fmaf(u1, u1, v1 * v1), e3) : 0.0f;
+                deposit(w.li, val0);
+                deposit(w.li + 1, val1);
+                w.step();
+            }
         } else {
             ChordWalk w;
             if (n > 0) w.start(t, g.tshift);
-            walk(w);
+            for (int p = 0; p < n; ++p) {
+                float u, v;
+                deposit(w.li, splat_val(eval_q(t.f, w.dx(t.f.fx), w.dy, u, v), e3));
+                w.step(t, g.tshift, p + 1 < n);
+            }
         }
     }
     __syncthreads();
@@ -1096,7 +1141,7 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_bwd_kernel(Geom g, Imag
                                                                     const float* __restrict__ grad,
                                                                     float4* __restrict__ partials,
                                                                     const uint32_t* __restrict__ order_in) {
-    extern __shared__ __align__(16) float gt[];  // T * T upstream gradient tile
+    extern __shared__ __align__(16) float gt[];  // T * T upstream gradient tile + 1 padding element
     const uint32_t s = blockIdx.x;
     if (s >= ctrl->nseg) return;
     const int4 d = segs[s];
@@ -1140,40 +1185,40 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_bwd_kernel(Geom g, Imag
             }
             const size_t slot = (size_t)d.y + j;
             const TileRec t = decode_tile_rec(recs, slot, st.X0, st.Y0, g.box_walk);
-            const int n = key < 255 ? key : (g.box_walk ? t.area : chord_pixels(t));
-            // Per pixel, with w = (1 - q) inside the footprint and 0 outside,
-            // accumulate  m0 = sum w^2 g  and the moments of  w g  over the pixel
-            // offsets (dx, dy):  m1 = sum w g dx, m2 = sum w g dy, m3 = sum w g dx^2,
-            // m4 = sum w g dx dy, m5 = sum w g dy^2.  With s = dk / r = c w g,
-            // c = -(12/pi) E, and (u, v) = A (dx, dy), the reference's sums
-            // (photon_differentials.cu:195-204) are linear in these moments.
-            float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f, m4 = 0.f, m5 = 0.f;
-            auto walk = [&](auto& cw) {
-                for (int p = 0; p < n; ++p) {
-                    const float dx = cw.dx(t.f.fx), dy = cw.dy;
-                    float u, v;
-                    const float q = eval_q(t.f, dx, dy, u, v);
-                    const float w = q < 1.0f ? 1.0f - q : 0.0f;
-                    const float wg = w * gt[cw.li];
-                    m0 = fmaf(w, wg, m0);
-                    const float px = wg * dx, py = wg * dy;
-                    m1 += px;
-                    m2 += py;
-                    m3 = fmaf(px, dx, m3);
-                    m4 = fmaf(px, dy, m4);
-                    m5 = fmaf(py, dy, m5);
-                    cw.step(t, g.tshift, p + 1 < n);
-                }
-            };
+            const int n = key < 255 ? key : (g.box_walk ? walk_pairs(t) : chord_pixels(t));
+            // The moments of the record's pixels (Moments).  With s = dk / r =
+            // c w g, c = -(12/pi) E, and (u, v) = A (dx, dy), the reference's
+            // sums (photon_differentials.cu:195-204) are linear in them.
+            Moments m;
             if (g.box_walk) {
-                BoxWalk cw;
+                PairWalk cw;
                 cw.start(t, g.tshift);
-                walk(cw);
+                for (int p = 0; p < n; ++p) {
+                    const float dy = cw.ky - t.f.fy;
+                    const float ru = t.f.a01 * dy, rv = t.f.a11 * dy;
+                    const float dx0 = cw.kx - t.f.fx, dx1 = (cw.kx + 1.0f) - t.f.fx;
+                    const float u0 = fmaf(t.f.a00, dx0, ru), v0 = fmaf(t.f.a10, dx0, rv);
+                    const float u1 = fmaf(t.f.a00, dx1, ru), v1 = fmaf(t.f.a10, dx1, rv);
+                    const float w0 = foot_w(fmaf(u0, u0, v0 * v0)), w1 = foot_w(fmaf(u1, u1, v1 * v1));
+                    // the second pixel's gradient may lie past the row (the
+                    // tile's padding element at the end): masked, not multiplied
+                    const float g0 = gt[cw.li], g1 = gt[cw.li + 1];
+                    m.add(w0, w0 * g0, dx0, dy);
+                    m.add(w1, cw.second() ? w1 * g1 : 0.0f, dx1, dy);
+                    cw.step();
+                }
             } else {
                 ChordWalk cw;
                 if (n > 0) cw.start(t, g.tshift);
-                walk(cw);
+                for (int p = 0; p < n; ++p) {
+                    const float dx = cw.dx(t.f.fx);
+                    float u, v;
+                    const float w = foot_w(eval_q(t.f, dx, cw.dy, u, v));
+                    m.add(w, w * gt[cw.li], dx, cw.dy);
+                    cw.step(t, g.tshift, p + 1 < n);
+                }
             }
+            const float m0 = m.m0, m1 = m.m1, m2 = m.m2, m3 = m.m3, m4 = m.m4, m5 = m.m5;
             const Frame& f = t.f;
             const float c = -kTwelveOverPi * t.e;
             const float su = c * fmaf(f.a00, m1, f.a01 * m2);  // sum s u
@@ -1439,7 +1484,8 @@ void launch_forward_tiles(const PhotonArgs& P, const ImageArgs& I, const Geom& g
 void launch_backward_tiles(const float* grad, const PhotonArgs& P, const ImageArgs& I, const Geom& g,
                            const Buffers& B, const GradArgs& G, bool sorted, hipStream_t s) {
     const unsigned nsegs = (unsigned)g.max_segments;
-    const size_t tile_lds = (size_t)g.T * g.T * 4;
+    // the gradient tile and one padding element (read by the pair walk, never used)
+    const size_t tile_lds = ((size_t)g.T * g.T + 1) * 4;
     if (sorted)
         hipLaunchKernelGGL(bin_tile_bwd_kernel<true>, dim3(nsegs), dim3(kTileThreads), tile_lds, s, g, I,
                            (const int4*)B.segs, (const uint4*)B.recs, (const Ctrl*)B.ctrl, grad, B.partials,
