@@ -1153,14 +1153,44 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_bwd_kernel(Geom g, Imag
     }
     const uint16_t* order = nullptr;
     const uint8_t* keys = nullptr;
+    constexpr int kWaves = kTileThreads / 64;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nchunks = (d.z + 63) >> 6;
+    // Each wavefront's first chunk is taken statically (the kWaves largest
+    // walks, one each), the rest from the queue, whose counter starts past them.
     __shared__ int wq;  // next_chunk's counter
-    if (threadIdx.x == 0) wq = 0;
+    if (threadIdx.x == 0) wq = kWaves;
+    const int first = wv < nchunks ? nchunks - 1 - wv : -1;
+    // one lane's record of a chunk: walk order entry (record index, -1 for
+    // lanes past the segment's end, and walk length key) and the record's two
+    // halves
+    auto load = [&](int ch, int& j, int& key, uint4& q0, uint4& q1) {
+        key = 0;
+        j = -1;
+        const int pos = (ch << 6) + lane;
+        if (ch < 0 || pos >= d.z) return;
+        if constexpr (SORTED) {
+            const uint32_t w = order_in[(size_t)d.y + pos];
+            j = (int)(w & 0xffffu);
+            key = (int)(w >> 16);
+        } else {
+            j = order[pos];
+            key = (int)keys[j];
+        }
+        q0 = recs[2 * ((size_t)d.y + j)];
+        q1 = recs[2 * ((size_t)d.y + j) + 1];
+    };
+    int ch = first, j = 0, key = 0;
+    uint4 q0 = make_uint4(0u, 0u, 0u, 0u), q1 = q0;
     if constexpr (!SORTED) {
         __shared__ SegSort S;
         sort_segment(S, recs, d, st.X0, st.Y0, g.box_walk);
         order = S.order;
         keys = S.key;
+        load(ch, j, key, q0, q1);
     } else {
+        // the first chunk's loads overlap the gradient tile's
+        load(ch, j, key, q0, q1);
         __syncthreads();
     }
     const float hw = 0.5f * I.fw, hh = 0.5f * I.fh, iw = 2.0f / I.fw, ih = 2.0f / I.fh;
@@ -1168,23 +1198,10 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_bwd_kernel(Geom g, Imag
     // request per record, as in the scatter)
     __shared__ float4 pst[kTileThreads / 64][2][64];
     __shared__ uint32_t pj[kTileThreads / 64][64];
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int nchunks = (d.z + 63) >> 6;
-    for (int ch; (ch = next_chunk(&wq, nchunks)) >= 0;) {
-        const int pos = (ch << 6) + lane;
-        const bool act = pos < d.z;
+    for (; ch >= 0; ch = next_chunk(&wq, nchunks), load(ch, j, key, q0, q1)) {
+        const bool act = j >= 0;
         if (act) {
-            int j, key;
-            if constexpr (SORTED) {
-                const uint32_t w = order_in[(size_t)d.y + pos];
-                j = (int)(w & 0xffffu);
-                key = (int)(w >> 16);
-            } else {
-                j = order[pos];
-                key = (int)keys[j];
-            }
-            const size_t slot = (size_t)d.y + j;
-            const TileRec t = decode_tile_rec(recs, slot, st.X0, st.Y0, g.box_walk);
+            const TileRec t = decode_tile_rec_q(q0, q1, st.X0, st.Y0, g.box_walk);
             const int n = key < 255 ? key : (g.box_walk ? walk_pairs(t) : chord_pixels(t));
             // The moments of the record's pixels (Moments).  With s = dk / r =
             // c w g, c = -(12/pi) E, and (u, v) = A (dx, dy), the reference's
