@@ -262,22 +262,23 @@ __device__ __forceinline__ BinPhoton bin_photon(const RawPhoton& R, const ImageA
 // 32-byte record of one (photon, tile) overlap, with the photon's footprint
 // box already clipped to the tile (the scatter has the box; the tile kernels
 // then neither recompute the window nor clip to the ellipse):
-//   x  fx quantised to 2^-23 px (signed, bits 8-31) | box x origin in the tile (0-5)
+//   x  fx quantised to 2^-25 px (signed, bits 6-31) | box x origin in the tile (0-5)
 //   y  fy likewise                                  | box y origin in the tile (0-5)
 //   z  energy (float bits)
 //   w  box width - 1 (0-5) | box height - 1 (6-11) | box x origin - ix + 64 (12-19) |
 //      box y origin - iy + 64 (20-27)
 //   second half: a00, a01, a10, a11 (float bits)
-// fx, fy lie in [-1/2, 1/2], so the quantised fraction is an exact float and
-// within 6e-8 px of the double fraction; box origins relative to (ix, iy)
-// lie in [-64, 64] (window radii <= 63).
+// fx, fy lie in [-1/2, 1/2], so the quantised fraction (|fxq| <= 2^24) is an
+// exact float, equal to the float fraction where |fx| >= 1/4 (whose spacing
+// is 2^-25 there) and within 1.5e-8 px of it below; box origins relative to
+// (ix, iy) lie in [-64, 64] (window radii <= 63).
 // ---------------------------------------------------------------------------
-constexpr float kFracScale = 8388608.0f;         // 2^23
-constexpr float kFracUnit = 1.0f / 8388608.0f;   // 2^-23
+constexpr float kFracScale = 33554432.0f;        // 2^25
+constexpr float kFracUnit = 1.0f / 33554432.0f;  // 2^-25
 
 __device__ __forceinline__ uint32_t quantise_frac(float f) { return (uint32_t)__float2int_rn(f * kFracScale); }
 
-__device__ __forceinline__ float frac_of(uint32_t word) { return (float)((int32_t)word >> 8) * kFracUnit; }
+__device__ __forceinline__ float frac_of(uint32_t word) { return (float)((int32_t)word >> 6) * kFracUnit; }
 
 // Record words x, y, w for the photon box [x0, x1] x [y0, y1] (already within
 // the image) in the tile with origin (X0, Y0), edge T; fxq, fyq from quantise_frac.
@@ -286,8 +287,8 @@ __device__ __forceinline__ uint4 rec_head(uint32_t fxq, uint32_t fyq, float e, i
     const int32_t bx0 = max(x0, X0), bx1 = min(x1, X0 + T - 1);
     const int32_t by0 = max(y0, Y0), by1 = min(y1, Y0 + T - 1);
     uint4 q;
-    q.x = (fxq << 8) | (uint32_t)(bx0 - X0);
-    q.y = (fyq << 8) | (uint32_t)(by0 - Y0);
+    q.x = (fxq << 6) | (uint32_t)(bx0 - X0);
+    q.y = (fyq << 6) | (uint32_t)(by0 - Y0);
     q.z = __float_as_uint(e);
     q.w = (uint32_t)(bx1 - bx0) | ((uint32_t)(by1 - by0) << 6) | ((uint32_t)(bx0 - ix + 64) << 12) |
           ((uint32_t)(by0 - iy + 64) << 20);
