@@ -371,6 +371,31 @@ def test_segment_sizes(seg):
 # ----------------------------------------------------------------------------- chord walk
 
 
+@pytest.mark.parametrize("mode", MODES)
+def test_centres_on_pixel_corners_and_half_pixels(mode):
+    # photon centres exactly on pixel corners, half-pixel offsets (the frame's
+    # fraction is then exactly 0 or +-1/2, the ends of the records' quantised
+    # range) and tile edges (multiples of 64 at a 128 x 128 image), thin and
+    # round footprints: exact in float at a power-of-two image size
+    rng = np.random.default_rng(17)
+    n, size, cap = 6000, (2, 128, 128), 10
+    px = rng.integers(-4, 2 * 128 + 8, n) / 2.0
+    py = rng.integers(-4, 2 * 128 + 8, n) / 2.0
+    px[:400] = rng.choice([63.5, 64.0, 64.5, 0.0, 127.5, 128.0], 400)
+    py[200:600] = rng.choice([63.5, 64.0, 64.5, 0.0, 127.5, 128.0], 400)
+    theta = rng.uniform(0, np.pi, n)
+    s1 = np.exp(rng.uniform(np.log(0.05), np.log(8.0), n))
+    s2 = s1 / rng.uniform(1.0, 12.0, n)
+    E = rng.uniform(0.5, 1.5, n) / n
+    p = WL.from_pixel_space(px, py, theta, s1, s2, E, np.arange(n) % 2, size, cap)
+    x = (p.xp.astype(np.float64) + 1.0) * 64.0
+    assert np.array_equal(x[0], px) and np.array_equal(x[1], py)      # the centres are exact
+    g = WL.upstream_grad(p.size, seed=18)
+    img, grads = _run(p, mode, g)
+    assert_image_close(img, O.forward_f64(*p.arrays(), p.size, cap), f"{mode} corner centres fwd")
+    assert_grads_close(grads, O.backward_f64(g, *p.arrays(), cap), f"{mode} corner centres bwd")
+
+
 def _thin_photons(n, size, cap, seed):
     """Extreme footprints: axes logU(0.01, 20) px, any rotation, so chords are
     often empty or a single pixel and boxes far larger than the ellipse."""
