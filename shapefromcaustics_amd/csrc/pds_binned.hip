@@ -15,7 +15,7 @@
 //   3. scatter   each block re-walks its photons and writes one 32-byte record
 //                per overlap at slot = binoff + atomicAdd_LDS(cursor[bin], 1);
 //                then clears the tiles of bins split over several segments
-//   4. tile      one workgroup per segment: area-sorted walk of its records,
+//   4. tile      one workgroup per segment: its records sorted by walk length,
 //                LDS accumulation, flush (store, or atomic add when split)
 //   5. convert   deterministic mode: int64 fixed point -> float for split bins
 //
@@ -798,8 +798,9 @@ __device__ __forceinline__ TileRec decode_tile_rec(const uint4* __restrict__ rec
     return decode_tile_rec_q(recs[2 * j], recs[2 * j + 1], X0, Y0, box_walk);
 }
 
-// Counting sort of a segment's records by their walk length (box area, or
-// the pixels of their chords; clamped to 255, the key is exact below that),
+// Counting sort of a segment's records by their walk length (steps of the
+// pair walk: box rows x pixel pairs per row; or the pixels of their chords;
+// clamped to 255, the key is exact below that),
 // so that the 64 records a wavefront walks side by side, one per lane, have
 // nearly the same length.  The box walk's length is in each record's w word:
 // that word is all this pass reads.
@@ -980,7 +981,7 @@ __device__ __forceinline__ void flat_splat_all(const PhotonArgs& P, const ImageA
 
 // ---------------------------------------------------------------------------
 // 4. tile (forward): one workgroup per segment; each lane splats whole
-//    records (area-sorted) into the LDS tile, which is then flushed.  On
+//    records (sorted by walk length) into the LDS tile, which is then flushed.  On
 //    overflow the workgroups run the photon-order walk instead.
 // ---------------------------------------------------------------------------
 
@@ -1131,7 +1132,7 @@ __global__ __launch_bounds__(kTileThreads) void bin_convert_split_kernel(Geom g,
 
 // ---------------------------------------------------------------------------
 // backward 1: tile adjoint.  The segment's gradient tile is staged in LDS; each
-// lane walks whole records (area-sorted), sums their seven terms in
+// lane walks whole records (sorted by walk length), sums their seven terms in
 // registers and stores them, in the reference's units, at the record's slot.
 // ---------------------------------------------------------------------------
 // SORTED: the forward of the same binning left each segment's walk order (and
