@@ -418,6 +418,7 @@ def main():
                     help="skip the batched, Landweber and scale-out sub-measurements of the default run")
     ap.add_argument("--eager", action="store_true", help="never use a HIP graph")
     ap.add_argument("--tuned", action="store_true", help="use the autotuner's cached tuning for this problem")
+    ap.add_argument("--tuning", metavar="JSON", help='explicit pds_tuning fields, e.g. \'{"layout": 1}\'')
     ap.add_argument("--summarize", metavar="DB", help="print a Markdown summary of a rocprofv3 result and exit")
     ap.add_argument("--fetch-factor", type=float, default=1.0, help="FETCH_SIZE calibration factor (--summarize)")
     ap.add_argument("--write-factor", type=float, default=1.0, help="WRITE_SIZE calibration factor (--summarize)")
@@ -509,7 +510,7 @@ def main():
         bwd = sorted(e[1].elapsed_time(e[2]) for e in evs.values())
         return wall, fwd[len(fwd) // 2] * 1e-3, bwd[len(bwd) // 2] * 1e-3
 
-    tuning = "cached" if args.tuned else None
+    tuning = "cached" if args.tuned else (json.loads(args.tuning) if args.tuning else None)
     graph = None
     if args.graph:
         # the whole step (forward keeping its bins + backward) captured once

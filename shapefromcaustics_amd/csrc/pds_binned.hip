@@ -12,6 +12,8 @@
 //                range inside the bin -> hist[G][nbins] = offset within bin
 //   2. plan      exclusive scan of totals, segment list (<= seg records each),
 //                capacity check, fixed-point scale for the deterministic mode
+//                (up to kFusedPlanBins bins: done by every scatter workgroup
+//                in its prologue instead of a launch of its own)
 //   3. scatter   each block re-walks its photons and writes one 32-byte record
 //                per overlap at slot = binoff + atomicAdd_LDS(cursor[bin], 1);
 //                then clears the tiles of bins split over several segments
@@ -29,7 +31,7 @@
 // If the records exceed the workspace capacity, the same launches switch to
 // the photon-order path (scatter zero-fills, tile runs the flattened walk with
 // global atomics / global int64 atomics, convert converts the whole image);
-// a device flag written by plan chooses, so the host never synchronises.
+// a device flag written by the plan chooses, so the host never synchronises.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -49,6 +51,10 @@ constexpr int kTileThreads = 512;
 constexpr int kMaxSeg = 2048;  // records per segment (sizes the LDS sort of a segment)
 constexpr int kSortKeys = 256;  // walk lengths 0..255 (clamped), sorted by one LDS counting sort
 constexpr int kMinBins = 256;
+// Up to this many bins, each scatter workgroup plans the segments itself (a
+// scan of the bin totals, a few microseconds of its prologue) instead of a
+// separate one-workgroup plan launch.
+constexpr int kFusedPlanBins = 1024;
 constexpr int64_t kDefaultBlockPhotons = 8192;     // photons per binning workgroup
 constexpr int64_t kMinDefaultBlockPhotons = 1024;  // ... at least, for small problems
 constexpr int64_t kTargetBinBlocks = 256;          // binning workgroups a small problem still gets (one per CU)
@@ -71,7 +77,8 @@ struct Geom {
     int32_t T, tshift, tiles_x, tiles_y, nbins, G, seg;
     int64_t per_block, cap_records, max_segments;
     bool det;
-    bool box_walk;  // walk whole footprint boxes (default) instead of the ellipse's row chords
+    bool box_walk;    // walk whole footprint boxes (default) instead of the ellipse's row chords
+    bool fused_plan;  // the scatter plans the segments (no plan launch)
 };
 
 struct Layout {
@@ -134,6 +141,7 @@ bool make_geom(int64_t N, int32_t C, int32_t H, int32_t W, int32_t cap, int32_t 
     g.max_segments = g.nbins + g.cap_records / g.seg + 1;
     g.det = mode == PDS_MODE_DETERMINISTIC;
     g.box_walk = !(t && t->walk == 1);
+    g.fused_plan = g.nbins <= kFusedPlanBins;
     return true;
 }
 
@@ -369,32 +377,41 @@ __global__ __launch_bounds__(kCountThreads) void bin_count_kernel(PhotonArgs P, 
 }
 
 // ---------------------------------------------------------------------------
-// 2. plan: one workgroup
+// 2. plan: exclusive scans of the bins' record totals and segment counts,
+//    segments of at most g.seg records (every bin gets one, so that its flush
+//    writes its tile), capacity check, fixed-point scale of the deterministic
+//    mode.  Run by one workgroup of NT threads: the plan launch, or each
+//    scatter workgroup for itself when the bins are few (fused_plan).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void bin_plan_kernel(Geom g, int64_t N, const uint32_t* __restrict__ totals,
-                                                        const float* __restrict__ emax_blk,
-                                                        uint32_t* __restrict__ binoff, int4* __restrict__ segs,
-                                                        Ctrl* ctrl) {
-    __shared__ uint64_t wsum[16];
-    __shared__ uint32_t wseg[16];
-    __shared__ float wmax[16];
-    __shared__ uint64_t base_rec;
-    __shared__ uint32_t base_seg;
+struct PlanLds {
+    uint64_t wsum[16];
+    uint32_t wseg[16];
+    float wmax[16];
+    uint64_t base_rec;  // after plan_scan: total records
+    uint32_t base_seg;  // ... and segments
+};
+
+// visit(bin, off, t, soff, ns): called by the thread holding bin, with its
+// first record, records, first segment and segments.
+template <int NT, class F>
+__device__ __forceinline__ void plan_scan(const Geom& g, const uint32_t* __restrict__ totals,
+                                          const float* __restrict__ emax_blk, PlanLds& L, F&& visit) {
+    static_assert(NT % 64 == 0 && NT <= 1024, "whole wavefronts, at most 16");
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     if (tid == 0) {
-        base_rec = 0;
-        base_seg = 0;
+        L.base_rec = 0;
+        L.base_seg = 0;
     }
     if (g.det) {
         // maximum finite positive energy over the binning blocks
         float m = 0.0f;
-        for (int k = tid; k < g.G; k += 1024) m = fmaxf(m, emax_blk[k]);
+        for (int k = tid; k < g.G; k += NT) m = fmaxf(m, emax_blk[k]);
         for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
-        if (lane == 0) wmax[w] = m;
+        if (lane == 0) L.wmax[w] = m;
     }
     __syncthreads();
-    // bins in chunks of 1024, one per thread, carried across chunks
-    for (int c0 = 0; c0 < g.nbins; c0 += 1024) {
+    // bins in chunks of NT, one per thread, carried across chunks
+    for (int c0 = 0; c0 < g.nbins; c0 += NT) {
         const int bin = c0 + tid;
         const uint32_t t = bin < g.nbins ? totals[bin] : 0u;
         const uint32_t ns = bin < g.nbins ? max(1u, (t + g.seg - 1) / g.seg) : 0u;
@@ -410,72 +427,114 @@ __global__ __launch_bounds__(1024) void bin_plan_kernel(Geom g, int64_t N, const
             }
         }
         if (lane == 63) {
-            wsum[w] = a;
-            wseg[w] = b;
+            L.wsum[w] = a;
+            L.wseg[w] = b;
         }
         __syncthreads();
-        uint64_t pa = base_rec;
-        uint32_t pb = base_seg;
+        uint64_t pa = L.base_rec;
+        uint32_t pb = L.base_seg;
         for (int k = 0; k < w; ++k) {
-            pa += wsum[k];
-            pb += wseg[k];
+            pa += L.wsum[k];
+            pb += L.wseg[k];
         }
-        const uint64_t off = pa + a - t;
-        const uint32_t soff = pb + b - ns;
-        if (bin < g.nbins) {
-            binoff[bin] = (uint32_t)off;
-            const bool fits = off + t <= (uint64_t)g.cap_records && soff + ns <= (uint64_t)g.max_segments;
-            if (fits)
-                for (uint32_t s = 0; s < ns; ++s) {
-                    const uint32_t st = (uint32_t)off + s * g.seg;
-                    segs[soff + s] = make_int4(bin, (int)st, (int)min((uint32_t)g.seg, t - s * g.seg),
-                                               (ns > 1 ? 1 : 0) | (s == 0 ? 2 : 0));
-                }
-        }
+        if (bin < g.nbins) visit(bin, pa + a - t, t, pb + b - ns, ns);
         __syncthreads();
-        if (tid == 1023) {
-            base_rec = pa + a;
-            base_seg = pb + b;
+        if (tid == NT - 1) {
+            L.base_rec = pa + a;
+            L.base_seg = pb + b;
         }
         __syncthreads();
     }
-    if (tid == 0) {
-        ctrl->total = base_rec;
-        const bool over = base_rec > (uint64_t)g.cap_records || base_seg > (uint64_t)g.max_segments;
-        ctrl->overflow = over ? 1u : 0u;
-        ctrl->nseg = over ? 0u : base_seg;
-        // fixed-point quantum: 2^e with N * max(E) * 3/pi <= 2^(e + 61)
-        float m = 0.0f;
-        if (g.det)
-            for (int k = 0; k < 16; ++k) m = fmaxf(m, wmax[k]);
-        ctrl->max_e_bits = __float_as_uint(m);
-        int e = 0;
-        if (m > 0.0f) {
-            const double bound = (double)N * (double)m * 0.9549296585513720;
-            e = (int)ceil(log2(bound)) - 61;
+}
+
+__device__ __forceinline__ bool plan_overflow(const Geom& g, const PlanLds& L) {
+    return L.base_rec > (uint64_t)g.cap_records || L.base_seg > (uint64_t)g.max_segments;
+}
+
+// The bin's segments, when they fit the workspace (else the plan overflows
+// and nothing reads them).
+__device__ __forceinline__ void write_segments(const Geom& g, int4* __restrict__ segs, int bin, uint64_t off,
+                                               uint32_t t, uint32_t soff, uint32_t ns) {
+    const bool fits = off + t <= (uint64_t)g.cap_records && soff + ns <= (uint64_t)g.max_segments;
+    if (fits)
+        for (uint32_t s = 0; s < ns; ++s) {
+            const uint32_t st = (uint32_t)off + s * g.seg;
+            segs[soff + s] = make_int4(bin, (int)st, (int)min((uint32_t)g.seg, t - s * g.seg),
+                                       (ns > 1 ? 1 : 0) | (s == 0 ? 2 : 0));
         }
-        ctrl->fx_exp = e;
+}
+
+// thread 0 of the planning workgroup, after plan_scan
+template <int NT>
+__device__ __forceinline__ void write_ctrl(const Geom& g, int64_t N, const PlanLds& L, Ctrl* ctrl) {
+    ctrl->total = L.base_rec;
+    const bool over = plan_overflow(g, L);
+    ctrl->overflow = over ? 1u : 0u;
+    ctrl->nseg = over ? 0u : L.base_seg;
+    // fixed-point quantum: 2^e with N * max(E) * 3/pi <= 2^(e + 61)
+    float m = 0.0f;
+    if (g.det)
+        for (int k = 0; k < NT / 64; ++k) m = fmaxf(m, L.wmax[k]);
+    ctrl->max_e_bits = __float_as_uint(m);
+    int e = 0;
+    if (m > 0.0f) {
+        const double bound = (double)N * (double)m * 0.9549296585513720;
+        e = (int)ceil(log2(bound)) - 61;
     }
+    ctrl->fx_exp = e;
+}
+
+__global__ __launch_bounds__(1024) void bin_plan_kernel(Geom g, int64_t N, const uint32_t* __restrict__ totals,
+                                                        const float* __restrict__ emax_blk,
+                                                        uint32_t* __restrict__ binoff, int4* __restrict__ segs,
+                                                        Ctrl* ctrl) {
+    __shared__ PlanLds L;
+    plan_scan<1024>(g, totals, emax_blk, L, [&](int bin, uint64_t off, uint32_t t, uint32_t soff, uint32_t ns) {
+        binoff[bin] = (uint32_t)off;
+        write_segments(g, segs, bin, off, t, soff, ns);
+    });
+    if (threadIdx.x == 0) write_ctrl<1024>(g, N, L, ctrl);
 }
 
 // ---------------------------------------------------------------------------
 // 3. scatter.  On overflow it zero-fills the output (or the int64 image) for
 //    the photon-order path instead.  Afterwards it clears the tiles of split
-//    bins, which the tile kernels then accumulate into atomically.
+//    bins, which the tile kernels then accumulate into atomically.  With
+//    fused_plan every workgroup first plans from the bin totals itself
+//    (binoff unused): the same scans give each workgroup its cursors, the
+//    workgroups write the segments of every gridDim.x-th bin, and workgroup 0
+//    the control block, which no workgroup of this launch reads.
 // ---------------------------------------------------------------------------
 template <bool KEEP>
 __global__ __launch_bounds__(kScatterThreads) void bin_scatter_kernel(PhotonArgs P, ImageArgs I, Geom g,
                                                                   const uint32_t* __restrict__ hist,
                                                                   const uint32_t* __restrict__ binoff,
-                                                                  const int4* __restrict__ segs,
+                                                                  const uint32_t* __restrict__ totals,
+                                                                  const float* __restrict__ emax_blk,
+                                                                  int4* __restrict__ segs,
                                                                   uint4* __restrict__ recs, uint32_t* __restrict__ slots,
-                                                                  uint8_t* __restrict__ info, const Ctrl* ctrl,
+                                                                  uint8_t* __restrict__ info, Ctrl* ctrl,
                                                                   float* __restrict__ out,
                                                                   unsigned long long* __restrict__ acc64) {
     extern __shared__ uint32_t cur[];  // nbins entries
+    const uint32_t* row = hist + (size_t)blockIdx.x * g.nbins;
+    bool overflow;
+    if (g.fused_plan) {
+        __shared__ PlanLds L;
+        plan_scan<kScatterThreads>(g, totals, emax_blk, L,
+                                   [&](int bin, uint64_t off, uint32_t t, uint32_t soff, uint32_t ns) {
+                                       cur[bin] = (uint32_t)off + row[bin];
+                                       if (bin % (int)gridDim.x == (int)blockIdx.x)
+                                           write_segments(g, segs, bin, off, t, soff, ns);
+                                   });
+        if (blockIdx.x == 0 && threadIdx.x == 0) write_ctrl<kScatterThreads>(g, P.N, L, ctrl);
+        overflow = plan_overflow(g, L);
+    } else {
+        overflow = ctrl->overflow != 0;
+    }
     // the image to clear: none when binning for a backward (no image output)
     const bool clear = g.det ? acc64 != nullptr : out != nullptr;
-    if (ctrl->overflow) {
+    if (overflow) {
         if (!clear) return;
         const int64_t n = (int64_t)I.C * I.H * I.W;
         const int64_t stride = (int64_t)gridDim.x * kScatterThreads;
@@ -487,9 +546,10 @@ __global__ __launch_bounds__(kScatterThreads) void bin_scatter_kernel(PhotonArgs
         }
         return;
     }
-    const uint32_t* row = hist + (size_t)blockIdx.x * g.nbins;
-    for (int k = threadIdx.x; k < g.nbins; k += kScatterThreads) cur[k] = binoff[k] + row[k];
-    __syncthreads();
+    if (!g.fused_plan) {
+        for (int k = threadIdx.x; k < g.nbins; k += kScatterThreads) cur[k] = binoff[k] + row[k];
+        __syncthreads();
+    }
     const int64_t lo = (int64_t)blockIdx.x * g.per_block;
     const int64_t hi = min(lo + g.per_block, live_photons(P));
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -551,14 +611,12 @@ __global__ __launch_bounds__(kScatterThreads) void bin_scatter_kernel(PhotonArgs
         }
         if (KEEP && valid) info[i] = (b.special || nrec > kMaxRecsPerPhoton) ? kSpecial : (uint8_t)nrec;
     }
-    // clear the tiles of split bins (first segment of each such bin)
+    // clear the tiles of split bins (more records than one segment holds)
     if (!clear) return;
-    const uint32_t nseg = ctrl->nseg;
     const int tpc = g.tiles_x * g.tiles_y;
-    for (uint32_t sg = blockIdx.x; sg < nseg; sg += gridDim.x) {
-        const int4 d = segs[sg];
-        if ((d.w & 3) != 3) continue;
-        const int c = d.x / tpc, t = d.x - c * tpc;
+    for (int bin = blockIdx.x; bin < g.nbins; bin += gridDim.x) {
+        if (totals[bin] <= (uint32_t)g.seg) continue;
+        const int c = bin / tpc, t = bin - c * tpc;
         const int ty = t / g.tiles_x, tx = t - ty * g.tiles_x;
         const int X0 = tx << g.tshift, Y0 = ty << g.tshift;
         for (int k = threadIdx.x; k < g.T * g.T; k += kScatterThreads) {
@@ -1468,16 +1526,13 @@ void launch_build(const PhotonArgs& P, const ImageArgs& I, const Geom& g, const 
     hipLaunchKernelGGL(bin_init_kernel, dim3(1), dim3(1024), 0, s, g, B.totals, B.ctrl);
     hipLaunchKernelGGL(bin_count_kernel, dim3(g.G), dim3(kCountThreads), hist_lds, s, P, I, g, B.hist, B.totals,
                        B.emax);
-    hipLaunchKernelGGL(bin_plan_kernel, dim3(1), dim3(1024), 0, s, g, P.N, (const uint32_t*)B.totals,
-                       (const float*)B.emax, B.binoff, B.segs, B.ctrl);
-    if (keep)
-        hipLaunchKernelGGL(bin_scatter_kernel<true>, dim3(g.G), dim3(kScatterThreads), hist_lds, s, P, I, g,
-                           (const uint32_t*)B.hist, (const uint32_t*)B.binoff, (const int4*)B.segs, B.recs, B.slots,
-                           B.info, (const Ctrl*)B.ctrl, out, B.acc64);
-    else
-        hipLaunchKernelGGL(bin_scatter_kernel<false>, dim3(g.G), dim3(kScatterThreads), hist_lds, s, P, I, g,
-                           (const uint32_t*)B.hist, (const uint32_t*)B.binoff, (const int4*)B.segs, B.recs,
-                           (uint32_t*)nullptr, (uint8_t*)nullptr, (const Ctrl*)B.ctrl, out, B.acc64);
+    if (!g.fused_plan)
+        hipLaunchKernelGGL(bin_plan_kernel, dim3(1), dim3(1024), 0, s, g, P.N, (const uint32_t*)B.totals,
+                           (const float*)B.emax, B.binoff, B.segs, B.ctrl);
+    hipLaunchKernelGGL(keep ? bin_scatter_kernel<true> : bin_scatter_kernel<false>, dim3(g.G), dim3(kScatterThreads),
+                       hist_lds, s, P, I, g, (const uint32_t*)B.hist, (const uint32_t*)B.binoff,
+                       (const uint32_t*)B.totals, (const float*)B.emax, B.segs, B.recs,
+                       keep ? B.slots : (uint32_t*)nullptr, keep ? B.info : (uint8_t*)nullptr, B.ctrl, out, B.acc64);
 }
 
 // tiles (-> convert); the same launches run the photon-order path on overflow
