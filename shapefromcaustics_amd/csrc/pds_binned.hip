@@ -208,17 +208,24 @@ struct RawPhoton {
     float e, cx, cy, radius, m00, m01, m02, m10, m11, m12;
 };
 
-// Streamed inputs: arrays a kernel reads once, in order, and nothing later in
+// Streamed data: arrays a kernel reads once, in order, and nothing later in
 // the pass reads again (the photons in the count and scatter passes, the
-// finalize's classes and slots) are loaded non-temporally, so that they do
-// not displace the lines the passes' scattered writes are still filling
-// (record runs, partials) from the caches.  Measured on the benchmark splat:
-// forward 229 -> 217 us with the photons, backward about 3 us less with the
-// classes and slots; non-temporal loads of gathered data (records, partials)
-// or non-temporal gradient stores were slower.
+// finalize's classes and slots) are loaded non-temporally, and the classes
+// and slots, which only the finalize reads, at the end of the backward, are
+// stored so too, so that they do not displace from the caches the lines that
+// the passes' scattered writes are still filling (record runs, partials).
+// Measured on the benchmark splat: forward 229 -> 217 us with the photons,
+// backward about 3 and 4 us less with the classes' and slots' loads and
+// stores; non-temporal loads of gathered data (records, partials) and
+// non-temporal stores of data read again soon (the walk order) or of the
+// gradients were slower.
 template <class T>
 __device__ __forceinline__ T stream_load(const T* p) {
     return __builtin_nontemporal_load(p);
+}
+template <class T>
+__device__ __forceinline__ void stream_store(T v, T* p) {
+    __builtin_nontemporal_store(v, p);
 }
 
 // with_e = false: the energy is not read (0); with_m2 = false: nor is M[:, 2]
@@ -607,7 +614,7 @@ __global__ __launch_bounds__(kScatterThreads) void bin_scatter_kernel(PhotonArgs
                                              ty << g.tshift, g.T);
                 stage[w][1][lane] = h1;
                 stage_slot[w][lane] = slot;
-                if (KEEP && k < kMaxRecsPerPhoton) slots[(size_t)k * P.N + i] = slot;
+                if (KEEP && k < kMaxRecsPerPhoton) stream_store(slot, &slots[(size_t)k * P.N + i]);
                 if (++kx == nx) {
                     kx = 0;
                     ++ky;
@@ -622,7 +629,7 @@ __global__ __launch_bounds__(kScatterThreads) void bin_scatter_kernel(PhotonArgs
             }
             wave_lds_fence();
         }
-        if (KEEP && valid) info[i] = (b.special || nrec > kMaxRecsPerPhoton) ? kSpecial : (uint8_t)nrec;
+        if (KEEP && valid) stream_store((b.special || nrec > kMaxRecsPerPhoton) ? kSpecial : (uint8_t)nrec, &info[i]);
     }
     // clear the tiles of split bins (more records than one segment holds)
     if (!clear) return;
