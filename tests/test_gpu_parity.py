@@ -232,6 +232,37 @@ def test_binned_tile_sizes_and_split_segments(mode, tile):
     assert_image_close(img, ref, f"{mode} T={tile}")
 
 
+def test_plan_in_scatter_and_plan_launch_agree():
+    # up to 1024 bins the scatter's workgroups plan the segments themselves;
+    # more bins take the separate plan launch.  4 x 256 x 136 at tile 8 has
+    # 4 x 32 x 17 = 2176 bins (plan launch), at tile 16 576 (plan in the
+    # scatter).  The photon count is no multiple of the 512-photon chunks the
+    # binning workgroups sweep, and 1024 photons per workgroup give 69
+    # workgroups of two chunk rounds, the second partial.  Deterministic
+    # images are the same integers summed, so bitwise equal across the two
+    # plans; both and their kept-bins gradients match the oracle.
+    from shapefromcaustics_amd.PhotonDifferentialSplatting import _backward, _bins_stats, _forward
+    p = WL.caustic(70001, (4, 256, 136), 12, seed=21)
+    g = WL.upstream_grad(p.size, seed=21)
+    E, xp, M, cp, radius = _dev(p)
+    gt = torch.from_numpy(g).cuda()
+    ref = O.forward_f64(*p.arrays(), p.size, p.cap)
+    idx = np.arange(0, p.n, 7)
+    ref_g = O.backward_f64(g, *p.arrays(), p.cap, idx=idx)
+    out = {}
+    for tile, bins_want in ((8, 2176), (16, 576)):
+        tun = {"tile": tile, "seg_photons": 256, "block_photons": 1024}
+        img, bins = _forward(E, xp, M, cp, radius, p.size, p.cap, "deterministic", tun, keep_bins=True)
+        st = _bins_stats(bins, p.n, p.size, p.cap, "deterministic", tun)
+        assert st["bins"] == bins_want and not st["overflow"]
+        assert_image_close(img, ref, f"tile {tile}")
+        gE, gx, gM = _backward(gt, E, xp, M, cp, radius, p.cap, "deterministic", tun, bins=bins)
+        ti = torch.from_numpy(idx).cuda()
+        assert_grads_close([gE[ti], gx[:, ti], gM[:, :, ti]], ref_g, f"tile {tile} gradients")
+        out[tile] = img
+    assert torch.equal(out[8], out[16])
+
+
 @pytest.mark.parametrize("mode", ["binned", "deterministic"])
 def test_forced_capacity_overflow_uses_photon_order_path(mode):
     p = WL.caustic(1 << 15, (3, 128, 128), 12, seed=3)
