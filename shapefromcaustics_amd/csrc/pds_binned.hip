@@ -24,9 +24,10 @@
 // Atomic mode accumulates in a double LDS tile (summation order varies run to
 // run; the flush rounds each pixel to float once).
 // Deterministic mode accumulates every contribution as a 64-bit fixed-point
-// integer (quantum 2^e with 2^e * 2^61 >= N max(E) 3/pi, far below FP32
-// resolution of any pixel), so the sum is exact and order-independent: the
-// output is bitwise reproducible.  +inf contributions set bit 62.
+// integer (quantum 2^e with 2^e * 2^61 >= N max(E) 3/pi and 2^e * 2^51 >=
+// max(E) 3/pi, far below FP32 resolution of any pixel), so the sum is exact
+// and order-independent: the output is bitwise reproducible.  +inf
+// contributions set bit 62.
 //
 // If the records exceed the workspace capacity, the same launches switch to
 // the photon-order path (scatter zero-fills, tile runs the flattened walk with
@@ -408,6 +409,14 @@ __global__ __launch_bounds__(kCountThreads) void bin_count_kernel(PhotonArgs P, 
     }
 }
 
+// Quantum exponent of the fixed point for N photons of maximal finite
+// positive energy m (the deterministic mode; see to_fixed): N m 3/pi <= 2^(e+61) and m 3/pi <= 2^(e+51).
+__device__ __forceinline__ int fixed_exponent(int64_t N, float m) {
+    if (!(m > 0.0f)) return 0;
+    const double peak = (double)m * 0.9549296585513720;
+    return max((int)ceil(log2((double)N * peak)) - 61, (int)ceil(log2(peak)) - 51);
+}
+
 // ---------------------------------------------------------------------------
 // 2. plan: exclusive scans of the bins' record totals and segment counts,
 //    segments of at most g.seg records (every bin gets one, so that its flush
@@ -503,17 +512,12 @@ __device__ __forceinline__ void write_ctrl(const Geom& g, int64_t N, const PlanL
     const bool over = plan_overflow(g, L);
     ctrl->overflow = over ? 1u : 0u;
     ctrl->nseg = over ? 0u : L.base_seg;
-    // fixed-point quantum: 2^e with N * max(E) * 3/pi <= 2^(e + 61)
+    // fixed-point quantum of the deterministic mode
     float m = 0.0f;
     if (g.det)
         for (int k = 0; k < NT / 64; ++k) m = fmaxf(m, L.wmax[k]);
     ctrl->max_e_bits = __float_as_uint(m);
-    int e = 0;
-    if (m > 0.0f) {
-        const double bound = (double)N * (double)m * 0.9549296585513720;
-        e = (int)ceil(log2(bound)) - 61;
-    }
-    ctrl->fx_exp = e;
+    ctrl->fx_exp = fixed_exponent(N, m);
 }
 
 __global__ __launch_bounds__(1024) void bin_plan_kernel(Geom g, int64_t N, const uint32_t* __restrict__ totals,
@@ -668,14 +672,34 @@ __global__ __launch_bounds__(kScatterThreads) void bin_scatter_kernel(PhotonArgs
 // ---------------------------------------------------------------------------
 // tile helpers
 // ---------------------------------------------------------------------------
+// The deterministic mode's fixed point.  Its quantum 2^e (fixed_exponent)
+// keeps every pixel's sum below 2^61 and every single contribution below
+// 2^51 quanta.  to_fixed(val) for val > 0 is the nearest multiple of 2^e (ties
+// to even): x = val 2^-e is exact in double and below 2^52 - 1, so adding 2^52
+// rounds it to an integer held in the low 52 bits of the sum -- a few
+// instructions instead of a 64-bit integer conversion.  +inf gives 0; its
+// pixel is marked by bit 62 instead (fixed_deposit).
+constexpr unsigned long long kInfMark = 1ull << 62;
+
 __device__ __forceinline__ unsigned long long to_fixed(float val, int e) {
-    // val in (0, 2^(e+61)]: quantise to the 2^e grid; +inf sets bit 62
-    if (!(val < INFINITY)) return 1ull << 62;
-    return (unsigned long long)__float2ll_rn(ldexpf(val, -e));
+    const double x = ldexp((double)val, -e);
+    return (unsigned long long)__double_as_longlong(x + 4503599627370496.0) & ((1ull << 52) - 1);
+}
+
+// Add the contribution val of a pixel (skipped unless val > 0, as the
+// reference's val > 0 test) to its fixed-point accumulator.  The integer
+// addition is exact and order-independent; a +inf value adds 0 and or-s the
+// marker in, which only wavefronts holding one execute.
+template <class Ptr>
+__device__ __forceinline__ void fixed_deposit(Ptr acc, float val, int e) {
+    if (!(val > 0.0f)) return;
+    atomicAdd(acc, to_fixed(val, e));
+    const bool inf = !(val < INFINITY);
+    if (__any(inf) && inf) atomicOr(acc, kInfMark);
 }
 
 __device__ __forceinline__ float from_fixed(unsigned long long v, int e) {
-    if (v & (1ull << 62)) return INFINITY;
+    if (v & kInfMark) return INFINITY;
     return (float)ldexp((double)(long long)v, e);
 }
 
@@ -1054,13 +1078,7 @@ __device__ __forceinline__ void flat_splat_all(const PhotonArgs& P, const ImageA
                     if (k < it.n) {
                         float u, v;
                         const float val = silverman(eval_q(it.f, px.dx[k], px.dy[k], u, v)) * it.e;
-                        if (val > 0.0f) {
-                            const unsigned long long fxv = to_fixed(val, fe);
-                            if (fxv >> 62)
-                                atomicOr(&acc64[px.pix[k]], fxv);
-                            else
-                                atomicAdd(&acc64[px.pix[k]], fxv);
-                        }
+                        fixed_deposit(&acc64[px.pix[k]], val, fe);
                     }
                 }
             } else {
@@ -1122,17 +1140,10 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_kernel(Geom g, PhotonAr
         const int n = key < 255 ? key : (g.box_walk ? walk_pairs(t) : chord_pixels(t));
         const float e3 = kThreeOverPi * t.e;
         auto deposit = [&](int li, float val) {
-            if (val > 0.0f) {
-                if constexpr (DET) {
-                    const unsigned long long fxv = to_fixed(val, fe);
-                    if (fxv >> 62)
-                        atomicOr(&tile[li], fxv);
-                    else
-                        atomicAdd(&tile[li], fxv);
-                } else {
-                    atomicAdd(&tile[li], (double)val);
-                }
-            }
+            if constexpr (DET)
+                fixed_deposit(&tile[li], val, fe);
+            else if (val > 0.0f)
+                atomicAdd(&tile[li], (double)val);
         };
         if (g.box_walk) {
             PairWalk w;
@@ -1173,8 +1184,8 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_kernel(Geom g, PhotonAr
                 // integer addition of the finite part (order-independent); the
                 // +inf marker is or-ed in, so that two segments' markers do
                 // not carry out of bit 62
-                if (v >> 62) atomicOr(&acc64[o], 1ull << 62);
-                const unsigned long long fin = v & ((1ull << 62) - 1);
+                if (v & kInfMark) atomicOr(&acc64[o], kInfMark);
+                const unsigned long long fin = v & (kInfMark - 1);
                 if (fin) atomicAdd(&acc64[o], fin);
             }
         } else {
@@ -1478,7 +1489,7 @@ __global__ __launch_bounds__(256) void emax_kernel(PhotonArgs P, ImageArgs I, Ct
 // guarded kernels run: overflow = 1)
 __global__ void fixed_point_kernel(Ctrl* ctrl, int64_t N) {
     const float m = __uint_as_float(ctrl->max_e_bits);
-    ctrl->fx_exp = m > 0.0f ? (int)ceil(log2((double)N * (double)m * 0.9549296585513720)) - 61 : 0;
+    ctrl->fx_exp = fixed_exponent(N, m);
     ctrl->overflow = 1;
 }
 
@@ -1501,13 +1512,7 @@ __global__ __launch_bounds__(256) void fwd_plain_fixed_kernel(PhotonArgs P, Imag
             const float dx = (float)((int32_t)px - f.ix) - f.fx;
             float u, v;
             const float val = silverman(eval_q(f, dx, dy, u, v)) * s.e;
-            if (val > 0.0f) {
-                const unsigned long long q = to_fixed(val, fe);
-                if (q >> 62)
-                    atomicOr(&plane[py * I.W + px], q);
-                else
-                    atomicAdd(&plane[py * I.W + px], q);
-            }
+            fixed_deposit(&plane[py * I.W + px], val, fe);
         }
     }
 }
