@@ -35,7 +35,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 #   forward : reads E 4 + xp 8 + M 24 + cp 8 + radius 4 = 48 B per photon, writes 4 B per pixel
 #   backward: reads the same 48 B, writes gE 4 + gx 8 + gM 24 = 36 B per photon, reads 4 B per pixel
 FWD_B_PER_PHOTON, BWD_B_PER_PHOTON, B_PER_PIXEL_PASS = 48, 84, 4
-EVENT_EVERY = 4  # timed steps per step that carries HIP events (see timed())
+EVENT_EVERY = 4  # timed steps per instrumented step sampled after them (see timed())
 
 
 def log(*a):
@@ -487,27 +487,29 @@ def main():
             ev[2].record()
         return img, g
 
-    def timed(run, steps, warmup, split=True, every=EVENT_EVERY):
+    def timed(run, steps, warmup, split=True):
         """run(ev) does one step, recording ev[0..2] (start, after forward, end)
-        when ev is not None.  Every `every`-th timed step carries the three
-        HIP events (each event record costs a few microseconds of stream
-        time, so instrumenting every step would inflate the wall time); the
-        pass durations are the medians over those steps."""
+        when ev is not None.  The wall clock times `steps` steps that carry no
+        events (an event record costs a few microseconds of stream time); the
+        pass durations are the medians of HIP events over steps // 4 (at least
+        3) further steps, run after the timed region."""
         for _ in range(warmup):
             run(None)
-        marked = set(range(0, steps, every))
-        evs = {k: [torch.cuda.Event(enable_timing=True) for _ in range(3)] for k in marked}
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for k in range(steps):
-            run(evs.get(k))
+        for _ in range(steps):
+            run(None)
         torch.cuda.synchronize()
         wall = (time.perf_counter() - t0) / steps
+        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(max(3, steps // EVENT_EVERY))]
+        for ev in evs:
+            run(ev)
+        torch.cuda.synchronize()
         if not split:
-            tot = sorted(e[0].elapsed_time(e[2]) for e in evs.values())
+            tot = sorted(e[0].elapsed_time(e[2]) for e in evs)
             return wall, tot[len(tot) // 2] * 1e-3, None
-        fwd = sorted(e[0].elapsed_time(e[1]) for e in evs.values())
-        bwd = sorted(e[1].elapsed_time(e[2]) for e in evs.values())
+        fwd = sorted(e[0].elapsed_time(e[1]) for e in evs)
+        bwd = sorted(e[1].elapsed_time(e[2]) for e in evs)
         return wall, fwd[len(fwd) // 2] * 1e-3, bwd[len(bwd) // 2] * 1e-3
 
     tuning = "cached" if args.tuned else (json.loads(args.tuning) if args.tuning else None)
