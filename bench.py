@@ -418,7 +418,7 @@ def main():
                     help="skip the batched, Landweber and scale-out sub-measurements of the default run")
     ap.add_argument("--eager", action="store_true", help="never use a HIP graph")
     ap.add_argument("--tuned", action="store_true", help="use the autotuner's cached tuning for this problem")
-    ap.add_argument("--tuning", metavar="JSON", help='explicit pds_tuning fields, e.g. \'{"layout": 1}\'')
+    ap.add_argument("--tuning", metavar="JSON", help='explicit pds_tuning fields, e.g. \'{"block_photons": 16384}\'')
     ap.add_argument("--summarize", metavar="DB", help="print a Markdown summary of a rocprofv3 result and exit")
     ap.add_argument("--fetch-factor", type=float, default=1.0, help="FETCH_SIZE calibration factor (--summarize)")
     ap.add_argument("--write-factor", type=float, default=1.0, help="WRITE_SIZE calibration factor (--summarize)")

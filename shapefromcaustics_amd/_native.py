@@ -101,6 +101,9 @@ def tuning_struct(tuning) -> "PdsTuning | None":
     if isinstance(tuning, PdsTuning):
         return tuning
     t = PdsTuning()
+    fields = {name for name, _ in PdsTuning._fields_ if name != "reserved"}
     for k, v in dict(tuning).items():
+        if k not in fields:
+            raise ValueError(f"unknown pds_tuning field {k!r} (fields: {', '.join(sorted(fields))})")
         setattr(t, k, int(v))
     return t

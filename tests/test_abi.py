@@ -168,6 +168,15 @@ def test_python_unknown_mode():
         PDSmod._mode("fastest")
 
 
+def test_tuning_fields_are_checked():
+    t = _native.tuning_struct({"tile": 32, "block_photons": 4096})
+    assert (t.tile, t.block_photons, t.seg_photons) == (32, 4096, 0)
+    with pytest.raises(ValueError, match="unknown pds_tuning field 'layout'"):
+        _native.tuning_struct({"layout": 1})
+    with pytest.raises(ValueError, match="unknown tuning"):
+        PDSmod._resolve_tuning("fastest", 1, (1, 8, 8), 4, "auto")
+
+
 def test_alias_module_name():
     import sys
 
