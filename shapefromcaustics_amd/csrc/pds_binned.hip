@@ -1095,13 +1095,15 @@ __device__ __forceinline__ void flat_splat_all(const PhotonArgs& P, const ImageA
 //    overflow the workgroups run the photon-order walk instead.
 // ---------------------------------------------------------------------------
 
+// Eight wavefronts per SIMD (four workgroups per CU, as the LDS allows): the
+// deterministic instance otherwise takes 104 SGPRs, which leaves room for
+// seven, i.e. three workgroups per CU; with the bound it spills 14 SGPRs into
+// VGPR lanes and its kernel time on the benchmark splat went 85.1 -> 78.0 us.
 template <bool DET>
-__global__ __launch_bounds__(kTileThreads) void bin_tile_kernel(Geom g, PhotonArgs P, ImageArgs I,
-                                                                const int4* __restrict__ segs,
-                                                                const uint4* __restrict__ recs, const Ctrl* ctrl,
-                                                                float* __restrict__ out,
-                                                                unsigned long long* __restrict__ acc64,
-                                                                uint32_t* __restrict__ order_out) {
+__global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8))) void bin_tile_kernel(
+    Geom g, PhotonArgs P, ImageArgs I, const int4* __restrict__ segs, const uint4* __restrict__ recs,
+    const Ctrl* ctrl, float* __restrict__ out, unsigned long long* __restrict__ acc64,
+    uint32_t* __restrict__ order_out) {
     // LDS accumulators: int64 fixed point (deterministic) or double.  Both
     // have fast LDS atomics on gfx950 (ds_add_u64, ds_add_f64), whereas the
     // single-precision ds_add_f32 serialises its lanes (about 3 cycles per
