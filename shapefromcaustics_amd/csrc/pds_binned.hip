@@ -58,6 +58,15 @@ constexpr int kMaxBins = 16384;
 constexpr int kMaxBinBlocks = 1024;
 constexpr int kTileThreads = 512;
 constexpr int kMaxSeg = 2048;  // records per segment (sizes the LDS sort of a segment)
+// The tile adjoint stages the partials of one part of kStageRecs of a
+// segment's records (by storage position) at a time in LDS (32 KB) and stores
+// them coalesced.  (Parts of 512 records, 16 KB, which let four workgroups
+// share a CU instead of three, measured slower: backward 176 against 160 us,
+// with twice the workgroup barriers and walk-order groups half the size.)
+constexpr int kStageRecs = 1024;
+constexpr int ilog2_c(int v) { return v <= 1 ? 0 : 1 + ilog2_c(v >> 1); }
+constexpr int kStageParts = kMaxSeg / kStageRecs;
+static_assert(kStageParts >= 1 && kStageParts <= 4 && kMaxSeg % kStageRecs == 0, "at most 4 parts (16-bit packed counts)");
 constexpr int kSortKeys = 256;  // walk lengths 0..255 (clamped), sorted by one LDS counting sort
 constexpr int kMinBins = 256;
 // Up to this many bins, each scatter workgroup plans the segments itself (a
@@ -996,6 +1005,48 @@ __device__ __forceinline__ void sort_segment(SegSort& S, const uint4* __restrict
     __syncthreads();
 }
 
+// The walk order a forward keeps for the tile adjoint: entries record |
+// key << 16, grouped by storage part (records j of part j / kStageRecs, in
+// order of the part), each group in walk order (a stable partition of the
+// sorted positions), so that the adjoint walks one part at a time and can
+// stage that part's partials in LDS.  Every part but the last is full, so part
+// p starts at p kStageRecs.  Each thread partitions kMaxSeg / kTileThreads
+// consecutive sorted positions; a workgroup scan of the threads' per-part
+// counts (16-bit fields of one 64-bit sum) places them.
+__device__ __forceinline__ void write_walk_order(const SegSort& S, const int4& d, uint32_t* __restrict__ order_out) {
+    constexpr int R = kMaxSeg / kTileThreads;
+    constexpr int kShift = ilog2_c(kStageRecs);
+    __shared__ int64_t wsum[kTileThreads / 64];
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    uint32_t ent[R];
+    int64_t cnt = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int pos = tid * R + r;
+        ent[r] = 0u;
+        if (pos < d.z) {
+            const uint32_t j = S.order[pos];
+            ent[r] = j | ((uint32_t)S.key[j] << 16);
+            cnt += (int64_t)1 << (16 * (j >> kShift));
+        }
+    }
+    const int64_t incl = wave_inclusive_sum(cnt);
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    int64_t rank = incl - cnt;  // per part: its entries at sorted positions before this thread's
+    for (int k = 0; k < w; ++k) rank += wsum[k];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int pos = tid * R + r;
+        if (pos < d.z) {
+            const int part = (int)((ent[r] & 0xffffu) >> kShift);
+            const int dst = part * kStageRecs + (int)((rank >> (16 * part)) & 0xffff);
+            order_out[(size_t)d.y + dst] = ent[r];
+            rank += (int64_t)1 << (16 * part);
+        }
+    }
+}
+
 // Wavefront work queue over a segment's sorted positions: chunks of 64
 // consecutive positions, handed out largest walks first (the sort is
 // ascending), so that the workgroup's wavefronts finish together.  A
@@ -1127,11 +1178,7 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8)
     for (int k = threadIdx.x; k < TT; k += kTileThreads) tile[k] = Acc(0);
     const SegTile st = seg_tile(g, I, d.x);
     sort_segment(S, recs, d, st.X0, st.Y0, g.box_walk);
-    if (order_out)  // keep the walk order (record | key << 16) for the backward
-        for (int pos = threadIdx.x; pos < d.z; pos += kTileThreads) {
-            const uint32_t j = S.order[pos];
-            order_out[(size_t)d.y + pos] = j | ((uint32_t)S.key[j] << 16);
-        }
+    if (order_out) write_walk_order(S, d, order_out);  // for the backward
     const int nchunks = (d.z + 63) >> 6;
     // (loading the next chunk's records while the current one walks measured
     // slower: 81 against 74 us, the registers they hold cost occupancy)
@@ -1292,7 +1339,12 @@ __device__ __forceinline__ void record_partial(const TileRec& t, int n, const fl
 // registers and stores them, in the reference's units, at the record's slot.
 // ---------------------------------------------------------------------------
 // SORTED: the forward of the same binning left each segment's walk order (and
-// every record's walk length) in order_in, so the segment is not sorted again.
+// every record's walk length) in order_in, grouped by storage part
+// (write_walk_order), so the segment is not sorted again, and the partials of
+// one part at a time are staged in LDS and then stored coalesced: stored
+// straight from the walk, at positions in walk order, they were scattered
+// 32-byte writes whose lines the L2 evicted half-written (backward pass
+// 165.5 -> 160.5 us on the benchmark splat, four alternating runs).
 template <bool SORTED>
 __global__ __launch_bounds__(kTileThreads) void bin_tile_bwd_kernel(Geom g, ImageArgs I, const int4* __restrict__ segs,
                                                                     const uint4* __restrict__ recs, const Ctrl* ctrl,
@@ -1309,69 +1361,84 @@ __global__ __launch_bounds__(kTileThreads) void bin_tile_bwd_kernel(Geom g, Imag
         const int px = st.X0 + (k & (T - 1)), py = st.Y0 + (k >> g.tshift);
         gt[k] = (px < I.W && py < I.H) ? grad[((int64_t)st.c * I.H + py) * I.W + px] : 0.0f;
     }
-    const uint16_t* order = nullptr;
-    const uint8_t* keys = nullptr;
     constexpr int kWaves = kTileThreads / 64;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int nchunks = (d.z + 63) >> 6;
-    // Each wavefront's first chunk is taken statically (the kWaves largest
-    // walks, one each), the rest from the queue, whose counter starts past them.
+    const float hw = 0.5f * I.fw, hh = 0.5f * I.fh, iw = 2.0f / I.fw, ih = 2.0f / I.fh;
     __shared__ int wq;  // next_chunk's counter
-    if (threadIdx.x == 0) wq = kWaves;
-    const int first = wv < nchunks ? nchunks - 1 - wv : -1;
-    // one lane's record of a chunk: walk order entry (record index, -1 for
-    // lanes past the segment's end, and walk length key) and the record's two
-    // halves
-    auto load = [&](int ch, int& j, int& key, uint4& q0, uint4& q1) {
-        key = 0;
-        j = -1;
-        const int pos = (ch << 6) + lane;
-        if (ch < 0 || pos >= d.z) return;
-        if constexpr (SORTED) {
-            const uint32_t w = order_in[(size_t)d.y + pos];
-            j = (int)(w & 0xffffu);
-            key = (int)(w >> 16);
-        } else {
-            j = order[pos];
-            key = (int)keys[j];
+    if constexpr (SORTED) {
+        __shared__ float4 stg[2 * kStageRecs];  // one storage part's partials
+        for (int part = 0; part < kStageParts; ++part) {
+            // order_in positions of this part: [base, base + cnt), records base + [0, cnt)
+            const int base = part * kStageRecs, cnt = min(d.z - base, kStageRecs);
+            if (cnt <= 0) break;
+            const int nch = (cnt + 63) >> 6;
+            // Each wavefront's first chunk is taken statically (the kWaves
+            // largest walks, one each), the rest from the queue, whose counter
+            // starts past them.
+            if (threadIdx.x == 0) wq = kWaves;
+            // the gradient tile is loaded (first part); the staged partials of
+            // the previous part are stored (later parts)
+            __syncthreads();
+            for (int ch = wv < nch ? nch - 1 - wv : -1; ch >= 0; ch = next_chunk(&wq, nch)) {
+                const int pos = (ch << 6) + lane;
+                if (pos >= cnt) continue;
+                const uint32_t w = order_in[(size_t)d.y + base + pos];
+                const int j = (int)(w & 0xffffu), key = (int)(w >> 16);
+                const TileRec t = decode_tile_rec_q(recs[2 * ((size_t)d.y + j)], recs[2 * ((size_t)d.y + j) + 1],
+                                                    st.X0, st.Y0, g.box_walk);
+                const int n = key < 255 ? key : (g.box_walk ? walk_pairs(t) : chord_pixels(t));
+                const int k = j - base;  // in [0, cnt): each once
+                record_partial(t, n, gt, TT - 1, g.tshift, g.box_walk, hw, hh, iw, ih, stg[2 * k], stg[2 * k + 1]);
+            }
+            __syncthreads();
+            float4* dst = partials + 2 * ((size_t)d.y + base);
+            for (int k = threadIdx.x; k < 2 * cnt; k += kTileThreads) dst[k] = stg[k];
         }
-        q0 = recs[2 * ((size_t)d.y + j)];
-        q1 = recs[2 * ((size_t)d.y + j) + 1];
-    };
-    int ch = first, j = 0, key = 0;
-    uint4 q0 = make_uint4(0u, 0u, 0u, 0u), q1 = q0;
-    if constexpr (!SORTED) {
+    } else {
+        const int nchunks = (d.z + 63) >> 6;
+        // Each wavefront's first chunk is taken statically (the kWaves largest
+        // walks, one each), the rest from the queue, whose counter starts past them.
+        if (threadIdx.x == 0) wq = kWaves;
+        const int first = wv < nchunks ? nchunks - 1 - wv : -1;
         __shared__ SegSort S;
         sort_segment(S, recs, d, st.X0, st.Y0, g.box_walk);
-        order = S.order;
-        keys = S.key;
+        // one lane's record of a chunk: record index (-1 for lanes past the
+        // segment's end), walk length key and the record's two halves
+        auto load = [&](int ch, int& j, int& key, uint4& q0, uint4& q1) {
+            key = 0;
+            j = -1;
+            const int pos = (ch << 6) + lane;
+            if (ch < 0 || pos >= d.z) return;
+            j = S.order[pos];
+            key = (int)S.key[j];
+            q0 = recs[2 * ((size_t)d.y + j)];
+            q1 = recs[2 * ((size_t)d.y + j) + 1];
+        };
+        int ch = first, j = 0, key = 0;
+        uint4 q0 = make_uint4(0u, 0u, 0u, 0u), q1 = q0;
         load(ch, j, key, q0, q1);
-    } else {
-        // the first chunk's loads overlap the gradient tile's
-        load(ch, j, key, q0, q1);
-        __syncthreads();
-    }
-    const float hw = 0.5f * I.fw, hh = 0.5f * I.fh, iw = 2.0f / I.fw, ih = 2.0f / I.fh;
-    // partials staged per wavefront and written by lane pairs (one 32-byte
-    // request per record, as in the scatter)
-    __shared__ float4 pst[kTileThreads / 64][2][64];
-    __shared__ uint32_t pj[kTileThreads / 64][64];
-    for (; ch >= 0; ch = next_chunk(&wq, nchunks), load(ch, j, key, q0, q1)) {
-        const bool act = j >= 0;
-        if (act) {
-            const TileRec t = decode_tile_rec_q(q0, q1, st.X0, st.Y0, g.box_walk);
-            const int n = key < 255 ? key : (g.box_walk ? walk_pairs(t) : chord_pixels(t));
-            record_partial(t, n, gt, TT - 1, g.tshift, g.box_walk, hw, hh, iw, ih, pst[wv][0][lane], pst[wv][1][lane]);
-            pj[wv][lane] = (uint32_t)j;
-        }
-        const uint64_t mask = __ballot(act);
-        wave_lds_fence();
+        // partials staged per wavefront and written by lane pairs (one 32-byte
+        // request per record, as in the scatter), at the records' positions
+        __shared__ float4 pst[kTileThreads / 64][2][64];
+        __shared__ uint32_t pj[kTileThreads / 64][64];
+        for (; ch >= 0; ch = next_chunk(&wq, nchunks), load(ch, j, key, q0, q1)) {
+            const bool act = j >= 0;
+            if (act) {
+                const TileRec t = decode_tile_rec_q(q0, q1, st.X0, st.Y0, g.box_walk);
+                const int n = key < 255 ? key : (g.box_walk ? walk_pairs(t) : chord_pixels(t));
+                record_partial(t, n, gt, TT - 1, g.tshift, g.box_walk, hw, hh, iw, ih, pst[wv][0][lane],
+                               pst[wv][1][lane]);
+                pj[wv][lane] = (uint32_t)j;
+            }
+            const uint64_t mask = __ballot(act);
+            wave_lds_fence();
 #pragma unroll
-        for (int p = 0; p < 2; ++p) {
-            const int src = (lane >> 1) + 32 * p, h = lane & 1;
-            if ((mask >> src) & 1ull) partials[2 * ((size_t)d.y + pj[wv][src]) + h] = pst[wv][h][src];
+            for (int p = 0; p < 2; ++p) {
+                const int src = (lane >> 1) + 32 * p, h = lane & 1;
+                if ((mask >> src) & 1ull) partials[2 * ((size_t)d.y + pj[wv][src]) + h] = pst[wv][h][src];
+            }
+            wave_lds_fence();
         }
-        wave_lds_fence();
     }
 }
 
