@@ -1150,6 +1150,9 @@ __device__ __forceinline__ void flat_splat_all(const PhotonArgs& P, const ImageA
 // deterministic instance otherwise takes 104 SGPRs, which leaves room for
 // seven, i.e. three workgroups per CU; with the bound it spills 14 SGPRs into
 // VGPR lanes and its kernel time on the benchmark splat went 85.1 -> 78.0 us.
+// The atomic instance fits eight as it is (100 SGPRs); bounded too, its code
+// fetches 56 MB more per launch (from the last-level cache, it seems) but the
+// forward measured ~3 us shorter than without the bound.
 template <bool DET>
 __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8))) void bin_tile_kernel(
     Geom g, PhotonArgs P, ImageArgs I, const int4* __restrict__ segs, const uint4* __restrict__ recs,
