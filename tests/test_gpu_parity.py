@@ -463,6 +463,34 @@ def test_chord_walk_equals_box_walk(case, tile):
 
 
 @pytest.mark.parametrize("mode", ["auto", "deterministic"])
+@pytest.mark.parametrize("seg", [1024, 1025, 2047, 2048])
+def test_kept_bins_backward_at_storage_half_boundaries(mode, seg):
+    # With kept bins the tile adjoint walks a segment's first 1024 records
+    # (by storage position), stores their partials, then walks the rest.
+    # Segments of 1024 records have no second half, of 1025 a second half of
+    # one record; full 64 px tiles of a caustic split into many segments of
+    # exactly seg records.  Gradients must equal the stateless backward's
+    # (which bins again, sorts and stores per record) bit for bit, and the
+    # oracle's.
+    from shapefromcaustics_amd.PhotonDifferentialSplatting import _backward, _bins_stats, _forward
+    p = WL.caustic(1 << 17, (3, 256, 256), 12, seed=seg)
+    g_np = WL.upstream_grad(p.size, seed=7)
+    g = torch.from_numpy(g_np).cuda()
+    E, xp, M, cp, radius = _dev(p)
+    tun = {"seg_photons": seg, "tile": 64}
+    img, bins = _forward(E, xp, M, cp, radius, p.size, p.cap, mode, tun, keep_bins=True)
+    stats = _bins_stats(bins, p.n, p.size, p.cap, mode, tun)
+    assert stats["seg_photons"] == seg and stats["tile"] == 64 and stats["segments"] > stats["bins"] + 8, stats
+    kept = _backward(g, E, xp, M, cp, radius, p.cap, mode, tun, bins=bins)
+    # (the same tiling: other tiles cut the footprints into other records)
+    fresh = pds_backward(g, E, xp, M, cp, radius, p.cap, mode=mode, tuning=tun)
+    for a, b in zip(kept, fresh):
+        assert torch.equal(a, b)
+    assert_image_close(img, O.forward_f64(*p.arrays(), p.size, p.cap), f"seg {seg} fwd")
+    assert_grads_close(kept, O.backward_f64(g_np, *p.arrays(), p.cap), f"seg {seg} bwd")
+
+
+@pytest.mark.parametrize("mode", ["auto", "deterministic"])
 def test_backward_with_kept_bins_equals_stateless(mode):
     # the backward that reuses the forward's bins (and its segment walk order)
     # must give exactly the gradients of the stateless backward, which bins and
