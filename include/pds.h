@@ -123,8 +123,11 @@ int pds_forward_noalloc_f32(const float* Ep, const float* xp, const float* Mp, c
  * The autograd path runs the forward and then the backward of the same
  * photons.  pds_forward_bins_f32 leaves the photons' tile binning in the
  * caller-owned ``bins`` buffer and pds_backward_bins_f32 reuses it instead of
- * binning again.  The inputs must be unchanged between the two calls.  No
- * counterpart in the reference (its backward re-walks every window). */
+ * binning again.  The inputs must be unchanged between the two calls, and so
+ * must N, C, H, W, max_pixel_radius, mode and the tuning: they fix the
+ * layout of ``bins`` (its segments, records and kept walk order), which the
+ * backward does not check.  No counterpart in the reference (its backward
+ * re-walks every window). */
 
 /* Bytes of ``bins`` for this problem; 0 if it cannot be binned (tiny image,
  * max_pixel_radius > 63, too many tiles, plain mode): use the plain pair. */
@@ -158,7 +161,8 @@ int pds_bins_stats_get(const void* bins, size_t bins_bytes, int64_t N, int32_t C
                        int32_t max_pixel_radius, int32_t mode, const pds_tuning* t, pds_bins_stats* out,
                        void* stream);
 
-/* pds_backward_f32 reusing ``bins`` from pds_forward_bins_f32 on the same inputs. */
+/* pds_backward_f32 reusing ``bins`` from pds_forward_bins_f32 on the same inputs
+ * and arguments (see above). */
 int pds_backward_bins_f32(const float* grad, const float* Ep, const float* xp, const float* Mp,
                           const int64_t* cp, const float* radius, int64_t N, int32_t C, int32_t H,
                           int32_t W, int32_t max_pixel_radius, int32_t mode, const pds_tuning* t,
